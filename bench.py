@@ -1,0 +1,161 @@
+"""Benchmark: exhaustive 3-way K2 search, sets x samples per second.
+
+Workload (default): 4096 SNPs x 262144 samples (131072 cases / 131072
+controls), HWE genotypes with MAF ~ U[0.05, 0.5], one planted triplet with a
+seeded 27-cell penetrance table (baseline 0.1, peak 0.4), seed 1 -- the shape
+of the reference's published run (README.md:54-60).  A step is one complete
+search over all C(4096,3) = 11,444,858,880 triplets with the data already
+resident in HBM: pair-table build, the fused persistent kernel, FP64
+re-scoring of candidates and the final reduction.
+
+value      = C(n,3) * (cases + controls) / seconds per step  (the reference's
+             metric, tensor-episdet.cu:198, unpadded samples)
+roofline   = the fused kernel alone, from HIP events on its stream:
+             achieved = 0.5 lane-ops per set-sample (VALU path: one v_and_b32
+             and one v_bcnt_u32_b32 per cell per 32 samples, 8 cells) x
+             set-samples / kernel seconds, against 256 CU x 64 lanes x 2.4 GHz
+plain      = the reference-shaped round loop (count kernel -> int32 tables in
+             HBM -> FP64 objective kernel), timed on its first R rounds and
+             scaled by the round count (all rounds have identical work)
+cpu        = the OpenMP AND+popcount oracle on a contiguous slice of the same
+             search (sets whose largest index lies in a stated range)
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--n SNPS] [--samples N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tensor-episdet_amd"))
+
+VALU_PEAK_OPS = 256 * 64 * 2.4e9        # 32-bit VALU lane-ops/s, MI355X
+OPS_PER_SET_SAMPLE = {2: 0.25, 3: 0.5}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--n", type=int, default=4096)
+    ap.add_argument("--samples", type=int, default=262144)
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--plain-rounds", type=int, default=2048)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the search shards with no "
+              "communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import episdet
+
+    n, N = args.n, args.samples
+    Nc = N // 2
+    Nt = N - Nc
+    order = args.order
+    t = time.time()
+    syn = episdet.generate(n, Nc, Nt, seed=args.seed, planted_sets=1, planted_order=order)
+    log(f"generated {n} SNPs x {N} samples in {time.time() - t:.1f} s; planted {syn.planted}")
+    name, cu, clk = episdet.device_info()
+    log(f"device {name}, {cu} CUs, {clk / 1e3:.0f} MHz")
+    ds = episdet.Dataset.from_synthetic(syn)
+    sets = episdet.n_choose_k(n, order)
+
+    def step():
+        return ds.run(order, episdet.K2, path=episdet.PATH_FUSED_VALU, tile=args.tile, rebuild=True)
+
+    for i in range(args.warmup):
+        r = step()
+        log(f"warmup {i}: {r['wall_s']:.3f} s, kernel {r['kernel_ms']:.1f} ms")
+    times, kms, results = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        r = step()
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        kms.append(r["kernel_ms"])
+        results.append(r)
+        log(f"step {i}: {dt:.3f} s (kernel {r['kernel_ms']:.1f} ms, pairs {r['prep_ms']:.1f} ms, rescore "
+            f"{r['rescore_ms']:.2f} ms, {r['candidates']} candidates) best {r['idx'][:order]} {r['best_score']:.6f}")
+    sec = max(times)
+    value = sets * N / sec
+    kernel_s = sum(kms) / len(kms) / 1e3
+    achieved = OPS_PER_SET_SAMPLE[order] * sets * N / kernel_s
+    r = results[-1]
+    planted_ok = tuple(r["idx"][:order]) == syn.planted[0]
+
+    plain = None
+    if not args.no_plain and order == 3:
+        rr = ds.run(order, episdet.K2, path=episdet.PATH_PLAIN, plain_rounds=args.plain_rounds)
+        total_rounds = rr["rounds"]
+        est = rr["kernel_ms"] / 1e3 * total_rounds / rr["rounds_run"]
+        plain = {"value": sets * N / est, "rounds_timed": rr["rounds_run"], "rounds_total": total_rounds,
+                 "seconds_est": est, "dtype": "f64 scoring", "speedup_fused": (sets * N / est) and value / (sets * N / est)}
+        log(f"plain: {rr['rounds_run']} rounds in {rr['kernel_ms']:.1f} ms -> {est:.1f} s estimated")
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = min(os.cpu_count() or 1, 16)
+        # calibrate a slice of the search (largest index in [z0, z1)) to about
+        # --cpu-seconds of work
+        z0 = max(order, min(n // 4, 600))
+        probe = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + 2))
+        t0 = time.perf_counter()
+        probe = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + 2))
+        rate = probe["sets"] / max(time.perf_counter() - t0, 1e-6)
+        per_z = probe["sets"] / 2
+        zn = int(max(2, min(n - z0, args.cpu_seconds * rate / per_z)))
+        t0 = time.perf_counter()
+        c = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + zn))
+        dt = time.perf_counter() - t0
+        cpu = {"value": c["sets"] * N / dt, "unit": "set-samples/s", "cores": threads, "kind": "port",
+               "sample": f"oracle (OpenMP AND+popcount{', AVX-512 VPOPCNTDQ' if oracle.uses_avx512() else ''}, "
+                         f"FP64 K2) on the {c['sets']} triplets whose largest index is in [{z0},{z0 + zn}), "
+                         f"{dt:.1f} s"}
+        log(f"cpu: {c['sets']} sets in {dt:.1f} s on {threads} threads")
+
+    out = {
+        "metric": "3-way K2 sets x samples/s (fused FP32-scored search, end to end, data resident in HBM)",
+        "value": value,
+        "unit": "set-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (bit-plane AND+popcount, exact int counts; FP32 scoring, FP64 re-score)",
+        "data": "synthetic",
+        "config": {"workload": f"3-way K2 exhaustive search, {n} SNPs x {N} samples ({Nc}/{Nt}), HWE, "
+                               f"MAF~U[0.05,0.5], 1 planted triplet, seed {args.seed}",
+                   "snps": n, "cases": Nc, "controls": Nt, "sets": sets, "path": "fused_valu",
+                   "tile": r["tile"], "parallelism": "1 GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12,
+                     "unit": "T lane-ops/s (32-bit VALU)", "frac": achieved / VALU_PEAK_OPS, "traffic": None,
+                     "kernel_ms": kernel_s * 1e3},
+        "cpu_baseline": cpu,
+        "plain_path": plain,
+        "best": {"idx": list(r["idx"][:order]), "k2": r["best_score"], "planted_recovered": planted_ok,
+                 "candidates": r["candidates"], "fp32_max_rel_err": r["fp32_max_rel_err"]},
+    }
+    print(json.dumps(out), flush=True)
+    ds.close()
+
+
+if __name__ == "__main__":
+    main()

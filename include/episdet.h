@@ -1,0 +1,162 @@
+/*
+ * episdet.h -- C ABI of the MI355X (gfx950) exhaustive epistasis search.
+ *
+ * Drop-in boundary for hiperbio/tensor-episdet's search entry point
+ *
+ *   cudaError_t EpistasisDetectionSearch(unsigned* cases, unsigned* controls,
+ *       int numSNPs, int numCases, int numControls, uint numSNPsWithPadding,
+ *       int numCasesWithPadding, int numControlsWithPadding, int* roundsCounter,
+ *       double* searchTime, float* outputFromGpu,
+ *       unsigned long long* packedIndices);
+ *                       (include/search-pairs.hpp:4, include/search-triplets.hpp:4)
+ *
+ * which the reference selects at build time (pairs/triplets x K2/MI,
+ * Makefile:8-22).  Here the interaction order and the score are arguments.
+ *
+ * Dataset layout (unchanged from the reference, tensor-episdet.cu:111-148):
+ * per class, u32 words [snp][plane][W]; plane 0 marks genotype 0, plane 1
+ * genotype 1, genotype 2 is neither; sample s is bit 31-(s%32) of word s/32;
+ * bits past the last sample are zero.  W is the padded word count per plane
+ * (the reference pads samples to a multiple of 1024, i.e. W = 32*ceil(N/1024)).
+ *
+ * Every function returns EPISDET_OK (0) or a positive episdet_status code and
+ * never throws.  The caller owns all host memory; the library owns all device
+ * memory.  A dataset handle may be used by one host thread at a time.
+ */
+#ifndef EPISDET_H_
+#define EPISDET_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum episdet_status {
+    EPISDET_OK = 0,
+    EPISDET_ERR_INVALID = 1,      /* bad argument or unsupported shape              */
+    EPISDET_ERR_NO_DEVICE = 2,    /* no gfx950 device visible                       */
+    EPISDET_ERR_HIP = 3,          /* a HIP runtime call failed                      */
+    EPISDET_ERR_ALLOC = 4,        /* device or host allocation failed               */
+    EPISDET_ERR_CANDIDATES = 5,   /* candidate buffer overflow that could not be
+                                     resolved by the exact fallback               */
+    EPISDET_ERR_INTERNAL = 6
+};
+
+enum episdet_score { EPISDET_SCORE_K2 = 0, EPISDET_SCORE_MI = 1 };
+
+enum episdet_path {
+    EPISDET_PATH_AUTO = 0,        /* library's choice                               */
+    EPISDET_PATH_FUSED_VALU = 1,  /* persistent count+score+reduce, AND+v_bcnt      */
+    EPISDET_PATH_FUSED_MFMA = 2,  /* persistent count+score+reduce, FP4 MFMA        */
+    EPISDET_PATH_PLAIN = 3        /* reference-shaped: host round loop, count kernel
+                                     writes int32 tables, FP64 objective kernel   */
+};
+
+/* Reference search, same twelve parameters and meaning as
+ * EpistasisDetectionSearch (search-pairs.hpp:4 / search-triplets.hpp:4),
+ * preceded by the interaction order (2 or 3) and the score.
+ *   cases, controls      host planes, rows [numSNPsWithPadding][2][W] with
+ *                        W = numCasesWithPadding/32 (resp. controls); rows at
+ *                        or beyond numSNPs are ignored.
+ *   roundsCounter        rounds the reference's host loop would run at
+ *                        BLOCK_SIZE=256 (search-triplets.cu:162-200), so its
+ *                        "Tensor TOPS" and "Ratio" lines stay comparable.
+ *   searchTime           wall-clock seconds including device allocation and
+ *                        host-to-device copies (search-triplets.cu:11-12,287-289).
+ *   outputFromGpu        best score (K2 minimised, MI maximised), FP64 value
+ *                        rounded to float.
+ *   packedIndices        X | Y<<21 | Z<<42 (3-way) or A | B<<32 (2-way),
+ *                        X < Y < Z; ties go to the lowest packed index. */
+int episdet_search(int order, int score, const unsigned *cases, const unsigned *controls, int numSNPs,
+                   int numCases, int numControls, unsigned numSNPsWithPadding, int numCasesWithPadding,
+                   int numControlsWithPadding, int *roundsCounter, double *searchTime, float *outputFromGpu,
+                   unsigned long long *packedIndices);
+
+/* ---- device-resident datasets (extended interface, no reference counterpart
+ * beyond the upload the reference does inside EpistasisDetectionSearch,
+ * search-triplets.cu:17-55) ---- */
+
+typedef struct episdet_dataset episdet_dataset;
+
+/* Uploads both classes to device 0 and keeps them resident.
+ * casesWords / controlsWords: W of the given host layout (>= ceil(N/32)). */
+int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int numSNPs, int numCases,
+                           int numControls, int casesWords, int controlsWords, episdet_dataset **out);
+void episdet_dataset_destroy(episdet_dataset *ds);
+
+typedef struct {
+    int order;            /* 2 or 3                                              */
+    int score;            /* episdet_score                                       */
+    int path;             /* episdet_path                                        */
+    int tile;             /* 0 = autotuned; otherwise a kernel tile id           */
+    int plain_rounds;     /* PLAIN only: stop after this many rounds (0 = all);
+                             used to time a contiguous slice of the round loop   */
+    int candidate_cap;    /* 0 = default (1<<20)                                 */
+    int rebuild;          /* 1: recompute the single and pair count tables inside
+                             this call, so their cost is part of it (the
+                             reference counts them inside its search,
+                             search-triplets.cu:113-117,173-197)                 */
+    int reserved[7];
+} episdet_options;
+
+typedef struct {
+    double best_score;          /* FP64 score of the winner                      */
+    float best_score_f32;       /* FP32 score the fused kernel gave the winner
+                                   (fused paths only; else equal to best_score)  */
+    int pad0;
+    unsigned long long packed;  /* winner, packed as above                       */
+    int idx[3];                 /* winner indices (idx[2] = -1 for pairs)        */
+    int path;                   /* path that ran                                 */
+    long long sets;             /* sets evaluated (C(n,k), or the slice's share) */
+    long long rounds;           /* reference-equivalent rounds                   */
+    long long rounds_run;       /* PLAIN: rounds actually launched               */
+    long long candidates;       /* fused: candidates re-scored in FP64           */
+    double kernel_ms;           /* device time of the main search kernel(s),
+                                   hipEvents on the launching stream             */
+    double rescore_ms;          /* device time of the FP64 re-scoring            */
+    double prep_ms;             /* host time of single/pair table builds in this
+                                   call (0 when cached)                          */
+    double wall_s;              /* host wall-clock of this call                  */
+    double fp32_max_abs_err;    /* max |FP32 - FP64| over re-scored candidates   */
+    double fp32_max_rel_err;    /* same, relative to |FP64|                      */
+    double margin;              /* FP32 candidate margin used (2 x error bound)  */
+    int fallback;               /* 1 if the exact fallback ran                   */
+    int tile;                   /* kernel tile id used                           */
+    int reserved[6];
+} episdet_result;
+
+int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result *res);
+
+/* Tables of explicit sets, counted on the device by the same counting and
+ * derivation code as the fused kernels.  sets: m x order indices, each row
+ * strictly increasing.  out: m x 2 x 3^order u32 (cases cells, then controls
+ * cells, cell index x*9+y*3+z or a*3+b). */
+int episdet_tables(episdet_dataset *ds, int order, const int *sets, int m, unsigned *out);
+
+/* Every set's table from the fused kernel itself (small n only):
+ * out[rank][2][3^order] in colex rank order, rank = C(Z,3)+C(Y,2)+X or
+ * C(B,2)+A; capacity in elements.  path: FUSED_VALU or FUSED_MFMA. */
+int episdet_dump_tables(episdet_dataset *ds, int order, int path, unsigned *out, size_t capacity);
+
+/* Order-independent 64-bit checksum over every set's table computed by the
+ * fused kernel (sum over sets of a hash of rank and the 2 x 3^k cells). */
+int episdet_checksum(episdet_dataset *ds, int order, int path, unsigned long long *checksum);
+
+/* Exact binomial C(n,k) (helper.cu:758-768). */
+unsigned long long episdet_n_choose_k(unsigned n, unsigned k);
+
+/* Rounds the reference's host loop runs for n SNPs at block size 256. */
+long long episdet_reference_rounds(int order, int numSNPs);
+
+const char *episdet_status_string(int status);
+
+/* Name, compute units and clock of device 0; EPISDET_ERR_NO_DEVICE if none. */
+int episdet_device_info(char *name, int name_len, int *compute_units, int *clock_khz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EPISDET_H_ */

@@ -1,0 +1,236 @@
+// device_common.hpp -- device-side building blocks shared by every search
+// kernel: contingency-cell derivation, FP32 scoring (fused paths), FP64
+// scoring (plain path and candidate re-scoring), order-preserving keys.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace episdet {
+
+enum { SCORE_K2 = 0, SCORE_MI = 1 };
+
+// Scoring context for the FP32 fused paths.
+struct ScoreCtx {
+    int score;          // SCORE_K2 / SCORE_MI
+    int ncells;         // 9 or 27
+    const float *t32;   // K2: ln(m!) - c*m (shifted, see k2_f32); MI: log2(m/N)
+    float k2_total;     // K2: ncells * c
+    float invN;         // MI: 1/N
+    float hY;           // MI: H(Y) in bits
+    float margin;       // candidate margin, 2 x the FP32 error bound
+};
+
+// Single-SNP counts of both classes: {cases c0, cases c1, controls c0,
+// controls c1}; genotype 2 is N - c0 - c1 (helper.cu:38-49).
+typedef uint4 Singles;
+
+// 3x3 table of the pair (a, b) from its four AND-counts n_ij = |a_i & b_j|,
+// i, j in {0,1}, and the single counts: the five derived cells of
+// helper.cu:644-662.  t[i*3+j].
+__device__ __forceinline__ void pair9(uint32_t n00, uint32_t n01, uint32_t n10, uint32_t n11, uint32_t a0,
+                                      uint32_t a1, uint32_t b0, uint32_t b1, uint32_t N, uint32_t t[9])
+{
+    t[0] = n00;
+    t[1] = n01;
+    t[3] = n10;
+    t[4] = n11;
+    t[2] = a0 - (n00 + n01);
+    t[5] = a1 - (n10 + n11);
+    t[6] = b0 - (n00 + n10);
+    t[7] = b1 - (n01 + n11);
+    t[8] = (N - a0 - a1) - (t[6] + t[7]);
+}
+
+#define EPI_C3(x, y, z) ((x) * 9 + (y) * 3 + (z))
+
+// 27-cell table from the eight AND-counts n8[i*4+j*2+k] = |X_i & Y_j & Z_k|
+// and the 3x3 pair tables xy, xz, yz.  Each of the 19 other cells is a pair
+// marginal minus two cells already known, in the order of helper.cu:493-548.
+__device__ __forceinline__ void triplet27(const uint32_t n8[8], const uint32_t xy[9], const uint32_t xz[9],
+                                          const uint32_t yz[9], uint32_t t[27])
+{
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int k = 0; k < 2; k++) t[EPI_C3(i, j, k)] = n8[i * 4 + j * 2 + k];
+    // genotype 2 of Z, X and Y in {0,1}
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) t[EPI_C3(i, j, 2)] = xy[i * 3 + j] - (t[EPI_C3(i, j, 0)] + t[EPI_C3(i, j, 1)]);
+    // genotype 2 of Y, X in {0,1}
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) t[EPI_C3(i, 2, k)] = xz[i * 3 + k] - (t[EPI_C3(i, 0, k)] + t[EPI_C3(i, 1, k)]);
+    // genotype 2 of X
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int k = 0; k < 2; k++) t[EPI_C3(2, j, k)] = yz[j * 3 + k] - (t[EPI_C3(0, j, k)] + t[EPI_C3(1, j, k)]);
+#pragma unroll
+    for (int j = 0; j < 2; j++) t[EPI_C3(2, j, 2)] = xy[2 * 3 + j] - (t[EPI_C3(2, j, 0)] + t[EPI_C3(2, j, 1)]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) t[EPI_C3(2, 2, k)] = xz[2 * 3 + k] - (t[EPI_C3(2, 0, k)] + t[EPI_C3(2, 1, k)]);
+}
+
+// ---------------------------------------------------------------- FP32 --
+
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v)
+{
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// K2 in FP32.  Each cell contributes f = ln ctl! + ln cas! - ln (ctl+cas+1)!
+// (scoring.hpp:5-7), always <= 0, so K2 = -sum f.  The table holds
+// T'(m) = ln m! - c*m with c = ln N - 1; since cas + ctl - (cas+ctl+1) = -1
+// every cell then reads f + c, and K2 = ncells*c - sum(f + c).  The shift
+// bounds |T'| by about N/e instead of N ln N, which is what keeps the FP32
+// error small; the cell sum is Kahan-compensated.
+__device__ __forceinline__ float k2_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, const float *T,
+                                        float total)
+{
+    float s = 0.f, c = 0.f;
+    for (int i = 0; i < ncells; i++) {
+        float f = (T[ctl[i]] + T[cas[i]]) - T[ctl[i] + cas[i] + 1];
+        kahan_add(s, c, f);
+    }
+    return total - s;
+}
+
+// MI in FP32: H(G) + H(Y) - H(G,Y) in bits (scoring.hpp:9-16), with
+// L(m) = log2(m/N) and compensated sums.
+__device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, const float *L,
+                                        float invN, float hY)
+{
+    float sG = 0.f, cG = 0.f, sGY = 0.f, cGY = 0.f;
+    for (int i = 0; i < ncells; i++) {
+        float pc = (float)cas[i] * invN, pt = (float)ctl[i] * invN;
+        kahan_add(sGY, cGY, pc * L[cas[i]]);
+        kahan_add(sGY, cGY, pt * L[ctl[i]]);
+        kahan_add(sG, cG, (pc + pt) * L[cas[i] + ctl[i]]);
+    }
+    // H(G) = -sG, H(G,Y) = -sGY
+    return (sGY - sG) + hY;
+}
+
+// "Badness": the FP32 score mapped so that lower is always better.
+__device__ __forceinline__ float badness_f32(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl)
+{
+    if (sc.score == SCORE_K2) return k2_f32(cas, ctl, sc.ncells, sc.t32, sc.k2_total);
+    return -mi_f32(cas, ctl, sc.ncells, sc.t32, sc.invN, sc.hY);
+}
+
+// ---------------------------------------------------------------- FP64 --
+// Fixed operation sequences, no contraction: identical to the oracle's.
+
+__device__ __forceinline__ double k2_f64(const uint32_t *cas, const uint32_t *ctl, int ncells, const double *T)
+{
+#pragma clang fp contract(off)
+    double s = 0.0;
+    for (int i = 0; i < ncells; i++) {
+        double t = T[ctl[i]] + T[cas[i]];
+        t = t - T[ctl[i] + cas[i] + 1];
+        s = s + t;
+    }
+    return fabs(s);
+}
+
+__device__ __forceinline__ double mi_f64(const uint32_t *cas, const uint32_t *ctl, int ncells, int Nc, int Nt,
+                                         const double *L)
+{
+#pragma clang fp contract(off)
+    double invN = 1.0 / (double)(Nc + Nt);
+    double hY = -((double)Nc * invN) * L[Nc];
+    hY = hY - ((double)Nt * invN) * L[Nt];
+    double hG = 0.0, hGY = 0.0;
+    for (int i = 0; i < ncells; i++) {
+        double pc = (double)cas[i] * invN;
+        double pt = (double)ctl[i] * invN;
+        hGY = hGY - pc * L[cas[i]];
+        hGY = hGY - pt * L[ctl[i]];
+        hG = hG - (pc + pt) * L[cas[i] + ctl[i]];
+    }
+    return (hG + hY) - hGY;
+}
+
+// ------------------------------------------------------------------ keys --
+
+// Order-preserving maps of floats/doubles to unsigned integers.
+__device__ __forceinline__ unsigned f2key(float f)
+{
+    unsigned u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(unsigned k)
+{
+    unsigned u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+__device__ __forceinline__ unsigned long long d2key(double d)
+{
+    unsigned long long u = (unsigned long long)__double_as_longlong(d);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double key2d(unsigned long long k)
+{
+    unsigned long long u = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min_d(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_sum_u(uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Colex ranks: C(Z,3) + C(Y,2) + X and C(B,2) + A.
+__device__ __forceinline__ unsigned long long rank3(int x, int y, int z)
+{
+    unsigned long long Z = (unsigned long long)z, Y = (unsigned long long)y;
+    return Z * (Z - 1) * (Z - 2) / 6 + Y * (Y - 1) / 2 + (unsigned long long)x;
+}
+__device__ __forceinline__ unsigned long long rank2(int a, int b)
+{
+    unsigned long long B = (unsigned long long)b;
+    return B * (B - 1) / 2 + (unsigned long long)a;
+}
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+// Hash of one set's table, combined by summation into an order-independent
+// checksum: h = (rank+1)*golden; h = mix64(h ^ cell) over cases then controls.
+__device__ __forceinline__ unsigned long long table_hash(unsigned long long rank, const uint32_t *cas,
+                                                         const uint32_t *ctl, int ncells)
+{
+    unsigned long long h = (rank + 1) * 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < ncells; i++) h = mix64(h ^ cas[i]);
+    for (int i = 0; i < ncells; i++) h = mix64(h ^ ctl[i]);
+    return h;
+}
+
+}  // namespace episdet

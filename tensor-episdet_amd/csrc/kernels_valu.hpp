@@ -1,0 +1,645 @@
+// kernels_valu.hpp -- gfx950 kernels that count with VALU AND + v_bcnt_u32_b32
+// over LDS-staged bit-plane tiles, plus the plain-path objective kernel and
+// the FP64 candidate re-scoring.
+//
+// Device layout of a class: u32 [nrows][plane][W], W a multiple of KW words,
+// bits past the last sample zero, rows >= n zero.  A 3-way work unit is one
+// SNP X, a tile of TY Y-SNPs and a tile of TZ Z-SNPs; a 2-way unit is a tile
+// pair.  Every unit streams its rows over the whole sample axis in KW-word
+// chunks staged through LDS; the counts of a unit stay in registers.
+#pragma once
+#include "device_common.hpp"
+
+namespace episdet {
+
+constexpr int KW = 32;          // words per staged chunk (1024 samples)
+constexpr int LDSW = KW + 4;    // LDS row stride in words: conflict-free b128 reads
+
+enum { MODE_SEARCH = 0, MODE_DUMP = 1, MODE_CHECKSUM = 2, MODE_COUNTS = 3, MODE_PAIRTABLE = 4 };
+
+// ------------------------------------------------------------ data prep --
+
+// Copies caller planes (rows of Win words) into the device layout (rows of
+// Wd words), clearing every bit at or past sample N and every SNP >= n.
+// dst holds nsnp_rows SNPs x 2 planes x Wd words; src holds n x 2 x Win.
+__global__ void k_repack(const uint32_t *__restrict__ src, int Win, uint32_t *__restrict__ dst, int Wd, int n,
+                         int nsnp_rows, int N)
+{
+    const size_t total = (size_t)nsnp_rows * 2 * Wd;
+    const int Wv = (N + 31) / 32;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        size_t row = i / Wd;
+        int w = (int)(i % Wd);
+        uint32_t v = 0;
+        if ((int)(row >> 1) < n && w < Wv && w < Win) {
+            v = src[row * Win + w];
+            int valid = N - w * 32;
+            if (valid < 32) v &= 0xFFFFFFFFu << (32 - valid);
+        }
+        dst[i] = v;
+    }
+}
+
+// Single-SNP plane counts: one wave per SNP, both classes
+// (helper.cu:13-52).  out[v] = {cases c0, cases c1, controls c0, controls c1}.
+__global__ __launch_bounds__(64) void k_singles(const uint32_t *__restrict__ p0, int W0,
+                                                const uint32_t *__restrict__ p1, int W1, int n, uint4 *out)
+{
+    const int v = blockIdx.x, lane = threadIdx.x;
+    if (v >= n) return;
+    uint32_t c[4] = {0, 0, 0, 0};
+    const uint32_t *r0 = p0 + (size_t)v * 2 * W0, *r1 = p1 + (size_t)v * 2 * W1;
+    for (int w = lane; w < W0; w += 64) {
+        c[0] += __popc(r0[w]);
+        c[1] += __popc(r0[W0 + w]);
+    }
+    for (int w = lane; w < W1; w += 64) {
+        c[2] += __popc(r1[w]);
+        c[3] += __popc(r1[W1 + w]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) c[i] = wave_sum_u(c[i]);
+    if (lane == 0) out[v] = make_uint4(c[0], c[1], c[2], c[3]);
+}
+
+// -------------------------------------------------------- shared output --
+
+struct CandOut {
+    unsigned *best_key;          // order-preserving key of the best FP32 badness
+    unsigned long long *cand;    // packed indices of candidates
+    float *cand_s32;             // their FP32 score
+    unsigned *cand_count;        // number appended (may exceed cand_cap: overflow)
+    unsigned cand_cap;
+};
+
+// Block-wide candidate publication for a work unit.  bad[] holds the
+// thread's FP32 badness values (+inf for invalid sets).  The block minimum
+// joins the global minimum; every set within `margin` of the smaller of the
+// two is appended.  Since the global minimum only decreases, the final
+// argmin of the exact scores is always among the appended sets when margin
+// is twice the FP32 error bound.
+template <int M>
+__device__ __forceinline__ void publish_candidates(const CandOut &co, float margin, const float (&bad)[M],
+                                                   const unsigned long long (&packed)[M], float *s_red)
+{
+    float tmin = INFINITY;
+#pragma unroll
+    for (int i = 0; i < M; i++) tmin = fminf(tmin, bad[i]);
+    float wmin = wave_min(tmin);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0) s_red[tid >> 6] = wmin;
+    __syncthreads();
+    if (tid == 0) {
+        float b = fminf(fminf(s_red[0], s_red[1]), fminf(s_red[2], s_red[3]));
+        float thr = -INFINITY;
+        if (b < INFINITY) {
+            unsigned old = atomicMin(co.best_key, f2key(b));
+            thr = fminf(key2f(old), b) + margin;
+        }
+        s_red[4] = thr;
+    }
+    __syncthreads();
+    const float thr = s_red[4];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        if (bad[i] <= thr) {
+            unsigned slot = atomicAdd(co.cand_count, 1u);
+            if (slot < co.cand_cap) {
+                co.cand[slot] = packed[i];
+                co.cand_s32[slot] = bad[i];
+            }
+        }
+    }
+}
+
+// ============================================================== 2-way ====
+
+struct PairArgs {
+    const uint32_t *p0, *p1;     // cases / controls planes
+    int W0, W1, n, N0, N1;
+    const uint4 *singles;
+    ScoreCtx sc;
+    int ntiles;                  // tiles along the SNP axis
+    unsigned long long nunits;   // ntiles*(ntiles+1)/2
+    unsigned long long *queue;
+    CandOut co;
+    uint4 *pairs;                // MODE_PAIRTABLE: [n][n][2] uint4 (a < b only)
+    uint32_t *dump;              // MODE_DUMP: [rank][2][9]
+    unsigned long long *checksum;
+};
+
+// Accumulates the four AND-counts of R x R pairs over one class: rows
+// a = a0 + ta + 16r, b = b0 + tb + 16s.
+template <int R>
+__device__ __forceinline__ void pair_count_class(const uint32_t *__restrict__ base, int W, int a0, int b0,
+                                                 uint32_t (*sA)[16 * R][LDSW], uint32_t (*sB)[16 * R][LDSW],
+                                                 uint32_t (&acc)[R][R][4])
+{
+    constexpr int T = 16 * R;
+    const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int s = 0; s < R; s++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[r][s][c] = 0;
+    for (int w0 = 0; w0 < W; w0 += KW) {
+        __syncthreads();
+        // 2*T rows for each side, KW/4 uint4 per row
+        for (int e = tid; e < 2 * T * (KW / 4); e += 256) {
+            int row = e / (KW / 4), q = e % (KW / 4);
+            int v = row >> 1, k = row & 1;
+            uint4 va = *(const uint4 *)(base + ((size_t)(a0 + v) * 2 + k) * W + w0 + q * 4);
+            uint4 vb = *(const uint4 *)(base + ((size_t)(b0 + v) * 2 + k) * W + w0 + q * 4);
+            *(uint4 *)&sA[k][v][q * 4] = va;
+            *(uint4 *)&sB[k][v][q * 4] = vb;
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int w = 0; w < KW; w += 4) {
+            uint4 av[R][2], bv[R][2];
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) av[r][k] = *(const uint4 *)&sA[k][ta + 16 * r][w];
+#pragma unroll
+            for (int s = 0; s < R; s++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) bv[s][k] = *(const uint4 *)&sB[k][tb + 16 * s][w];
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int s = 0; s < R; s++)
+#pragma unroll
+                    for (int i = 0; i < 2; i++)
+#pragma unroll
+                        for (int j = 0; j < 2; j++) {
+                            uint32_t c = acc[r][s][i * 2 + j];
+                            c += __popc(av[r][i].x & bv[s][j].x);
+                            c += __popc(av[r][i].y & bv[s][j].y);
+                            c += __popc(av[r][i].z & bv[s][j].z);
+                            c += __popc(av[r][i].w & bv[s][j].w);
+                            acc[r][s][i * 2 + j] = c;
+                        }
+        }
+    }
+}
+
+__device__ __forceinline__ void unrank_tri(unsigned long long u, int &ta, int &tb)
+{
+    // u = tb*(tb+1)/2 + ta, 0 <= ta <= tb
+    double d = sqrt(8.0 * (double)u + 1.0);
+    long long b = (long long)((d - 1.0) * 0.5);
+    while ((unsigned long long)(b + 1) * (unsigned long long)(b + 2) / 2 <= u) b++;
+    while ((unsigned long long)b * (unsigned long long)(b + 1) / 2 > u) b--;
+    tb = (int)b;
+    ta = (int)(u - (unsigned long long)b * (unsigned long long)(b + 1) / 2);
+}
+
+template <int R, int MODE>
+__global__ __launch_bounds__(256) void k_pairs_valu(PairArgs p)
+{
+    constexpr int T = 16 * R;
+    __shared__ uint32_t sA[2][T][LDSW];
+    __shared__ uint32_t sB[2][T][LDSW];
+    __shared__ unsigned long long s_u;
+    __shared__ float s_red[8];
+    const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
+    for (;;) {
+        if (tid == 0) s_u = atomicAdd(p.queue, 1ull);
+        __syncthreads();
+        const unsigned long long u = s_u;
+        if (u >= p.nunits) break;
+        int tA, tB;
+        unrank_tri(u, tA, tB);
+        const int a0 = tA * T, b0 = tB * T;
+        uint32_t acc0[R][R][4], acc1[R][R][4];
+        pair_count_class<R>(p.p0, p.W0, a0, b0, sA, sB, acc0);
+        pair_count_class<R>(p.p1, p.W1, a0, b0, sA, sB, acc1);
+
+        float bad[R * R];
+        unsigned long long packed[R * R];
+        unsigned long long hsum = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int A = a0 + ta + 16 * r;
+#pragma unroll
+            for (int s = 0; s < R; s++) {
+                const int B = b0 + tb + 16 * s;
+                const int m = r * R + s;
+                bad[m] = INFINITY;
+                packed[m] = (unsigned long long)A | ((unsigned long long)B << 32);
+                if (!(A < B && B < p.n)) continue;
+                if constexpr (MODE == MODE_PAIRTABLE) {
+                    uint4 *o = p.pairs + ((size_t)A * p.n + B) * 2;
+                    o[0] = make_uint4(acc0[r][s][0], acc0[r][s][1], acc0[r][s][2], acc0[r][s][3]);
+                    o[1] = make_uint4(acc1[r][s][0], acc1[r][s][1], acc1[r][s][2], acc1[r][s][3]);
+                } else {
+                    const Singles sa = p.singles[A], sb = p.singles[B];
+                    uint32_t cas[9], ctl[9];
+                    pair9(acc0[r][s][0], acc0[r][s][1], acc0[r][s][2], acc0[r][s][3], sa.x, sa.y, sb.x, sb.y,
+                          (uint32_t)p.N0, cas);
+                    pair9(acc1[r][s][0], acc1[r][s][1], acc1[r][s][2], acc1[r][s][3], sa.z, sa.w, sb.z, sb.w,
+                          (uint32_t)p.N1, ctl);
+                    if constexpr (MODE == MODE_SEARCH) {
+                        bad[m] = badness_f32(p.sc, cas, ctl);
+                    } else if constexpr (MODE == MODE_DUMP) {
+                        uint32_t *o = p.dump + rank2(A, B) * 18;
+                        for (int c = 0; c < 9; c++) {
+                            o[c] = cas[c];
+                            o[9 + c] = ctl[c];
+                        }
+                    } else if constexpr (MODE == MODE_CHECKSUM) {
+                        hsum += table_hash(rank2(A, B), cas, ctl, 9);
+                    }
+                }
+            }
+        }
+        if constexpr (MODE == MODE_SEARCH) publish_candidates<R * R>(p.co, p.sc.margin, bad, packed, s_red);
+        if constexpr (MODE == MODE_CHECKSUM) {
+            if (hsum) atomicAdd(p.checksum, hsum);
+        }
+    }
+}
+
+// ============================================================== 3-way ====
+
+struct TriArgs {
+    const uint32_t *p0, *p1;
+    int W0, W1, n, N0, N1;
+    const uint4 *singles;
+    const uint4 *pairs;                  // [n][n][2] uint4, a < b
+    ScoreCtx sc;
+    const unsigned long long *gstart;    // [ngroups+1] first unit of each group
+    const int2 *groups;                  // (Y tile, Z tile)
+    int ngroups;
+    unsigned long long nunits;
+    unsigned long long *queue;
+    CandOut co;
+    uint32_t *dump;                      // MODE_DUMP: [rank][2][27]
+    unsigned long long *checksum;
+    // MODE_COUNTS (plain path): one round = X, Y block yb, Z block zb of 256
+    int X, yb, zb;
+    uint32_t *counts;                    // [2][256][256][8]
+};
+
+__device__ __forceinline__ int find_group(const unsigned long long *gs, int ng, unsigned long long u)
+{
+    int lo = 0, hi = ng;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (gs[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Eight AND-counts of A x B (Y, Z) combinations with a fixed X over one
+// class.  P_ij = X_i & Y_j is formed while staging, so the inner loop is one
+// v_and_b32 + one v_bcnt_u32_b32 per cell per word.
+template <int A, int B>
+__device__ __forceinline__ void tri_count_class(const uint32_t *__restrict__ base, int W, int X, int y0, int z0,
+                                                uint32_t (*sP)[16 * A][LDSW], uint32_t (*sZ)[16 * B][LDSW],
+                                                uint32_t (&acc)[A][B][8])
+{
+    constexpr int TY = 16 * A, TZ = 16 * B;
+    const int tid = threadIdx.x, ty = tid >> 4, tz = tid & 15;
+#pragma unroll
+    for (int a = 0; a < A; a++)
+#pragma unroll
+        for (int b = 0; b < B; b++)
+#pragma unroll
+            for (int c = 0; c < 8; c++) acc[a][b][c] = 0;
+    const uint32_t *xr = base + (size_t)X * 2 * W;
+    for (int w0 = 0; w0 < W; w0 += KW) {
+        __syncthreads();
+        for (int e = tid; e < TY * KW; e += 256) {
+            const int y = e / KW, w = e % KW;
+            const uint32_t *yr = base + (size_t)(y0 + y) * 2 * W + w0 + w;
+            const uint32_t x0 = xr[w0 + w], x1 = xr[W + w0 + w];
+            const uint32_t v0 = yr[0], v1 = yr[W];
+            sP[0][y][w] = x0 & v0;
+            sP[1][y][w] = x0 & v1;
+            sP[2][y][w] = x1 & v0;
+            sP[3][y][w] = x1 & v1;
+        }
+        for (int e = tid; e < 2 * TZ * (KW / 4); e += 256) {
+            const int row = e / (KW / 4), q = e % (KW / 4);
+            const int z = row >> 1, k = row & 1;
+            *(uint4 *)&sZ[k][z][q * 4] = *(const uint4 *)(base + ((size_t)(z0 + z) * 2 + k) * W + w0 + q * 4);
+        }
+        __syncthreads();
+#pragma unroll 2
+        for (int w = 0; w < KW; w += 4) {
+            uint4 pv[A][4], zv[B][2];
+#pragma unroll
+            for (int a = 0; a < A; a++)
+#pragma unroll
+                for (int ij = 0; ij < 4; ij++) pv[a][ij] = *(const uint4 *)&sP[ij][ty + 16 * a][w];
+#pragma unroll
+            for (int b = 0; b < B; b++)
+#pragma unroll
+                for (int k = 0; k < 2; k++) zv[b][k] = *(const uint4 *)&sZ[k][tz + 16 * b][w];
+#pragma unroll
+            for (int a = 0; a < A; a++)
+#pragma unroll
+                for (int b = 0; b < B; b++)
+#pragma unroll
+                    for (int ij = 0; ij < 4; ij++)
+#pragma unroll
+                        for (int k = 0; k < 2; k++) {
+                            uint32_t c = acc[a][b][ij * 2 + k];
+                            c += __popc(pv[a][ij].x & zv[b][k].x);
+                            c += __popc(pv[a][ij].y & zv[b][k].y);
+                            c += __popc(pv[a][ij].z & zv[b][k].z);
+                            c += __popc(pv[a][ij].w & zv[b][k].w);
+                            acc[a][b][ij * 2 + k] = c;
+                        }
+        }
+    }
+}
+
+// 3x3 pair table of (a, b), a < b, class cls, from the pair AND-count table.
+__device__ __forceinline__ void pair9_lookup(const uint4 *pairs, int n, const Singles &sa, const Singles &sb, int a,
+                                             int b, int cls, uint32_t N, uint32_t t[9])
+{
+    const uint4 q = pairs[((size_t)a * n + b) * 2 + cls];
+    if (cls == 0) pair9(q.x, q.y, q.z, q.w, sa.x, sa.y, sb.x, sb.y, N, t);
+    else pair9(q.x, q.y, q.z, q.w, sa.z, sa.w, sb.z, sb.w, N, t);
+}
+
+// Both classes' 27-cell tables of (X, Y, Z) from its counts.
+__device__ __forceinline__ void triplet_tables(const TriArgs &p, int X, int Y, int Z, const Singles &sx,
+                                               const Singles &sy, const uint32_t n8c[8], const uint32_t n8t[8],
+                                               uint32_t cas[27], uint32_t ctl[27])
+{
+    const Singles sz = p.singles[Z];
+    uint32_t xy[9], xz[9], yz[9];
+    pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 0, (uint32_t)p.N0, xy);
+    pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 0, (uint32_t)p.N0, xz);
+    pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 0, (uint32_t)p.N0, yz);
+    triplet27(n8c, xy, xz, yz, cas);
+    pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 1, (uint32_t)p.N1, xy);
+    pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 1, (uint32_t)p.N1, xz);
+    pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 1, (uint32_t)p.N1, yz);
+    triplet27(n8t, xy, xz, yz, ctl);
+}
+
+template <int A, int B, int MODE>
+__global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
+{
+    constexpr int TY = 16 * A, TZ = 16 * B;
+    __shared__ uint32_t sP[4][TY][LDSW];
+    __shared__ uint32_t sZ[2][TZ][LDSW];
+    __shared__ unsigned long long s_u;
+    __shared__ float s_red[8];
+    const int tid = threadIdx.x, ty = tid >> 4, tz = tid & 15;
+    for (int iter = 0;; iter++) {
+        int X, y0, z0;
+        if constexpr (MODE == MODE_COUNTS) {
+            if (iter > 0) break;
+            const int tiles_z = 256 / TZ;
+            X = p.X;
+            y0 = p.yb * 256 + (int)(blockIdx.x / tiles_z) * TY;
+            z0 = p.zb * 256 + (int)(blockIdx.x % tiles_z) * TZ;
+        } else {
+            if (tid == 0) s_u = atomicAdd(p.queue, 1ull);
+            __syncthreads();
+            const unsigned long long u = s_u;
+            if (u >= p.nunits) break;
+            const int g = find_group(p.gstart, p.ngroups, u);
+            X = (int)(u - p.gstart[g]);
+            const int2 gt = p.groups[g];
+            y0 = gt.x * TY;
+            z0 = gt.y * TZ;
+        }
+        uint32_t acc0[A][B][8], acc1[A][B][8];
+        tri_count_class<A, B>(p.p0, p.W0, X, y0, z0, sP, sZ, acc0);
+        tri_count_class<A, B>(p.p1, p.W1, X, y0, z0, sP, sZ, acc1);
+
+        if constexpr (MODE == MODE_COUNTS) {
+#pragma unroll
+            for (int a = 0; a < A; a++)
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    const int yl = y0 - p.yb * 256 + ty + 16 * a, zl = z0 - p.zb * 256 + tz + 16 * b;
+                    uint4 *o0 = (uint4 *)(p.counts + ((size_t)(0 * 256 + yl) * 256 + zl) * 8);
+                    uint4 *o1 = (uint4 *)(p.counts + ((size_t)(1 * 256 + yl) * 256 + zl) * 8);
+                    o0[0] = make_uint4(acc0[a][b][0], acc0[a][b][1], acc0[a][b][2], acc0[a][b][3]);
+                    o0[1] = make_uint4(acc0[a][b][4], acc0[a][b][5], acc0[a][b][6], acc0[a][b][7]);
+                    o1[0] = make_uint4(acc1[a][b][0], acc1[a][b][1], acc1[a][b][2], acc1[a][b][3]);
+                    o1[1] = make_uint4(acc1[a][b][4], acc1[a][b][5], acc1[a][b][6], acc1[a][b][7]);
+                }
+        } else {
+            const Singles sx = p.singles[X];
+            float bad[A * B];
+            unsigned long long packed[A * B];
+            unsigned long long hsum = 0;
+#pragma unroll
+            for (int a = 0; a < A; a++) {
+                const int Y = y0 + ty + 16 * a;
+                const bool vy = X < Y && Y < p.n;
+                const Singles sy = vy ? p.singles[Y] : Singles{0, 0, 0, 0};
+#pragma unroll
+                for (int b = 0; b < B; b++) {
+                    const int Z = z0 + tz + 16 * b;
+                    const int m = a * B + b;
+                    bad[m] = INFINITY;
+                    packed[m] = (unsigned long long)X | ((unsigned long long)Y << 21) | ((unsigned long long)Z << 42);
+                    if (!(vy && Y < Z && Z < p.n)) continue;
+                    uint32_t cas[27], ctl[27];
+                    triplet_tables(p, X, Y, Z, sx, sy, acc0[a][b], acc1[a][b], cas, ctl);
+                    if constexpr (MODE == MODE_SEARCH) {
+                        bad[m] = badness_f32(p.sc, cas, ctl);
+                    } else if constexpr (MODE == MODE_DUMP) {
+                        uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
+                        for (int c = 0; c < 27; c++) {
+                            o[c] = cas[c];
+                            o[27 + c] = ctl[c];
+                        }
+                    } else if constexpr (MODE == MODE_CHECKSUM) {
+                        hsum += table_hash(rank3(X, Y, Z), cas, ctl, 27);
+                    }
+                }
+            }
+            if constexpr (MODE == MODE_SEARCH) publish_candidates<A * B>(p.co, p.sc.margin, bad, packed, s_red);
+            if constexpr (MODE == MODE_CHECKSUM) {
+                if (hsum) atomicAdd(p.checksum, hsum);
+            }
+        }
+    }
+}
+
+// ==================================================== plain objective ====
+
+struct PlainObjArgs {
+    const uint32_t *counts;              // [2][256][256][8] from MODE_COUNTS
+    const uint4 *singles;
+    const uint4 *pairs;
+    int n, N0, N1, score;
+    const double *t64;                   // FP64 table (ln m! or log2(m/N))
+    int X, yb, zb;
+    unsigned long long *best_key;        // key of the best FP64 badness
+    unsigned long long *cand;            // packed
+    double *cand_s64;
+    unsigned *cand_count;
+    unsigned cand_cap;
+};
+
+// Derives all 27 cells from the round's count tables, scores in FP64, and
+// publishes every set whose exact score ties or beats the running best
+// (helper.cu:438-584, reshaped: one thread per (Y, Z), exact scores, no
+// nondeterministic index race).
+__global__ __launch_bounds__(256) void k_triplets_objective_plain(PlainObjArgs p)
+{
+    __shared__ double s_red[8];
+    const int yl = blockIdx.x, zl = threadIdx.x;
+    const int X = p.X, Y = p.yb * 256 + yl, Z = p.zb * 256 + zl;
+    const bool valid = X < Y && Y < Z && Z < p.n;
+    double bad = INFINITY;
+    if (valid) {
+        const uint32_t *c0 = p.counts + ((size_t)(0 * 256 + yl) * 256 + zl) * 8;
+        const uint32_t *c1 = p.counts + ((size_t)(1 * 256 + yl) * 256 + zl) * 8;
+        uint32_t n8c[8], n8t[8];
+        for (int i = 0; i < 8; i++) {
+            n8c[i] = c0[i];
+            n8t[i] = c1[i];
+        }
+        const Singles sx = p.singles[X], sy = p.singles[Y], sz = p.singles[Z];
+        uint32_t xy[9], xz[9], yz[9], cas[27], ctl[27];
+        pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 0, (uint32_t)p.N0, xy);
+        pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 0, (uint32_t)p.N0, xz);
+        pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 0, (uint32_t)p.N0, yz);
+        triplet27(n8c, xy, xz, yz, cas);
+        pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 1, (uint32_t)p.N1, xy);
+        pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 1, (uint32_t)p.N1, xz);
+        pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 1, (uint32_t)p.N1, yz);
+        triplet27(n8t, xy, xz, yz, ctl);
+        bad = p.score == SCORE_K2 ? k2_f64(cas, ctl, 27, p.t64) : -mi_f64(cas, ctl, 27, p.N0, p.N1, p.t64);
+    }
+    const double wmin = wave_min_d(bad);
+    const int tid = threadIdx.x;
+    if ((tid & 63) == 0) s_red[tid >> 6] = wmin;
+    __syncthreads();
+    if (tid == 0) {
+        double b = fmin(fmin(s_red[0], s_red[1]), fmin(s_red[2], s_red[3]));
+        double thr = -INFINITY;
+        if (b < INFINITY) {
+            unsigned long long old = atomicMin(p.best_key, d2key(b));
+            thr = fmin(key2d(old), b);
+        }
+        s_red[4] = thr;
+    }
+    __syncthreads();
+    if (valid && bad <= s_red[4]) {
+        unsigned slot = atomicAdd(p.cand_count, 1u);
+        if (slot < p.cand_cap) {
+            p.cand[slot] = (unsigned long long)X | ((unsigned long long)Y << 21) | ((unsigned long long)Z << 42);
+            p.cand_s64[slot] = bad;
+        }
+    }
+}
+
+// ======================================================= FP64 rescoring ==
+
+struct RescoreArgs {
+    const uint32_t *p0, *p1;
+    int W0, W1, n, N0, N1, order, score;
+    const uint4 *singles;
+    const uint4 *pairs;                 // 3-way only
+    const double *t64;
+    const unsigned long long *cand;     // packed sets
+    int count;
+    double *out_s64;                    // FP64 score (not badness), may be null
+    uint32_t *out_cells;                // [count][2][3^k], may be null
+};
+
+// One wave per set: counts the 2^k AND-cells of both classes straight from
+// the planes, derives every cell, scores in FP64.
+__global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)
+{
+    const int i = blockIdx.x, lane = threadIdx.x;
+    if (i >= p.count) return;
+    const unsigned long long pk = p.cand[i];
+    int s[3];
+    if (p.order == 3) {
+        s[0] = (int)(pk & 0x1FFFFF);
+        s[1] = (int)((pk >> 21) & 0x1FFFFF);
+        s[2] = (int)((pk >> 42) & 0x1FFFFF);
+    } else {
+        s[0] = (int)(pk & 0xFFFFFFFFu);
+        s[1] = (int)(pk >> 32);
+        s[2] = 0;
+    }
+    uint32_t n8[2][8];
+    for (int cls = 0; cls < 2; cls++) {
+        const uint32_t *base = cls ? p.p1 : p.p0;
+        const int W = cls ? p.W1 : p.W0;
+        uint32_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const uint32_t *r0 = base + (size_t)s[0] * 2 * W, *r1 = base + (size_t)s[1] * 2 * W;
+        const uint32_t *r2 = base + (size_t)s[2] * 2 * W;
+        for (int w = lane; w < W; w += 64) {
+            if (p.order == 3) {
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++)
+#pragma unroll
+                        for (int z = 0; z < 2; z++)
+                            c[x * 4 + y * 2 + z] += __popc(r0[x * W + w] & r1[y * W + w] & r2[z * W + w]);
+            } else {
+#pragma unroll
+                for (int x = 0; x < 2; x++)
+#pragma unroll
+                    for (int y = 0; y < 2; y++) c[x * 2 + y] += __popc(r0[x * W + w] & r1[y * W + w]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) n8[cls][k] = wave_sum_u(c[k]);
+    }
+    if (lane != 0) return;
+    uint32_t cas[27], ctl[27];
+    int nc;
+    if (p.order == 3) {
+        nc = 27;
+        const Singles sx = p.singles[s[0]], sy = p.singles[s[1]], sz = p.singles[s[2]];
+        uint32_t xy[9], xz[9], yz[9];
+        pair9_lookup(p.pairs, p.n, sx, sy, s[0], s[1], 0, (uint32_t)p.N0, xy);
+        pair9_lookup(p.pairs, p.n, sx, sz, s[0], s[2], 0, (uint32_t)p.N0, xz);
+        pair9_lookup(p.pairs, p.n, sy, sz, s[1], s[2], 0, (uint32_t)p.N0, yz);
+        triplet27(n8[0], xy, xz, yz, cas);
+        pair9_lookup(p.pairs, p.n, sx, sy, s[0], s[1], 1, (uint32_t)p.N1, xy);
+        pair9_lookup(p.pairs, p.n, sx, sz, s[0], s[2], 1, (uint32_t)p.N1, xz);
+        pair9_lookup(p.pairs, p.n, sy, sz, s[1], s[2], 1, (uint32_t)p.N1, yz);
+        triplet27(n8[1], xy, xz, yz, ctl);
+    } else {
+        nc = 9;
+        const Singles sa = p.singles[s[0]], sb = p.singles[s[1]];
+        pair9(n8[0][0], n8[0][1], n8[0][2], n8[0][3], sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
+        pair9(n8[1][0], n8[1][1], n8[1][2], n8[1][3], sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);
+    }
+    if (p.out_cells) {
+        uint32_t *o = p.out_cells + (size_t)i * 2 * nc;
+        for (int c = 0; c < nc; c++) {
+            o[c] = cas[c];
+            o[nc + c] = ctl[c];
+        }
+    }
+    if (p.out_s64)
+        p.out_s64[i] = p.score == SCORE_K2 ? k2_f64(cas, ctl, nc, p.t64) : mi_f64(cas, ctl, nc, p.N0, p.N1, p.t64);
+}
+
+// Small fill kernels (no hipMemsetAsync, so the sequence stays capturable).
+__global__ void k_init_search(unsigned long long *queue, unsigned *best_key, unsigned *cand_count,
+                              unsigned long long *best_key64, unsigned long long *checksum)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *queue = 0ull;
+        *best_key = 0xFF800000u;                 // key of +inf
+        *cand_count = 0u;
+        *best_key64 = 0xFFF0000000000000ull;     // key of +inf (double)
+        *checksum = 0ull;
+    }
+}
+
+}  // namespace episdet

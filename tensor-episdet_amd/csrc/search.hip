@@ -1,0 +1,954 @@
+// search.hip -- host driver and C ABI (include/episdet.h) of the gfx950
+// epistasis search.
+//
+// The reference's EpistasisDetectionSearch (search-pairs.cu:9-246,
+// search-triplets.cu:8-305) runs a host loop of rounds, each a binary GEMM
+// plus an objective kernel.  Here the default path is one persistent launch
+// that walks the whole set space from a device work queue and fuses counting,
+// FP32 scoring and candidate selection; candidates are then re-scored in FP64
+// and reduced on the host with a deterministic tie rule.  The reference-shaped
+// round loop is kept as the PLAIN path (and as the exact fallback).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "episdet.h"
+#include "kernels_valu.hpp"
+
+using namespace episdet;
+
+namespace {
+
+constexpr int BLOCK = 256;            // the reference's BLOCK_SIZE (Makefile:2)
+constexpr int NUM_STREAMS = 8;        // the reference's NUM_STREAMS (Makefile:2)
+constexpr unsigned DEFAULT_CAP = 1u << 20;
+
+#define HIPCHK(expr)                                                                                    \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) {                                                                         \
+            std::fprintf(stderr, "episdet: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_),      \
+                         __FILE__, __LINE__);                                                           \
+            return EPISDET_ERR_HIP;                                                                     \
+        }                                                                                               \
+    } while (0)
+
+#define CHK(expr)                                                                                       \
+    do {                                                                                                \
+        int r_ = (expr);                                                                                \
+        if (r_ != EPISDET_OK) return r_;                                                                \
+    } while (0)
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct ScoreTables {
+    bool ready = false;
+    float *t32 = nullptr;
+    double *t64 = nullptr;
+    double t32_max = 0;       // max |t32| (K2)
+    double shift_c = 0;       // K2 shift per cell
+};
+
+struct Ctl {                  // device control block
+    unsigned long long queue;
+    unsigned long long best64;
+    unsigned long long checksum;
+    unsigned best32;
+    unsigned cand_count;
+};
+
+}  // namespace
+
+struct episdet_dataset {
+    int dev = 0;
+    int n = 0, nrows = 0, N[2] = {0, 0}, W[2] = {0, 0};
+    uint32_t *planes[2] = {nullptr, nullptr};
+    uint4 *singles = nullptr;
+    uint4 *pairs = nullptr;
+    ScoreTables tab[2];
+    hipStream_t stream = nullptr;
+    Ctl *ctl = nullptr;
+    unsigned long long *cand = nullptr;
+    float *cand_s32 = nullptr;
+    double *cand_s64 = nullptr;
+    unsigned cand_cap = 0;
+    int cu = 0;
+};
+
+namespace {
+
+int check_device(int *cu)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return EPISDET_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, 0));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        std::fprintf(stderr, "episdet: device 0 is %s, gfx950 required\n", prop.gcnArchName);
+        return EPISDET_ERR_NO_DEVICE;
+    }
+    if (cu) *cu = prop.multiProcessorCount;
+    return EPISDET_OK;
+}
+
+int ensure_cand(episdet_dataset *ds, unsigned cap)
+{
+    if (ds->cand_cap >= cap) return EPISDET_OK;
+    if (ds->cand) (void)hipFree(ds->cand);
+    if (ds->cand_s32) (void)hipFree(ds->cand_s32);
+    if (ds->cand_s64) (void)hipFree(ds->cand_s64);
+    ds->cand = nullptr;
+    ds->cand_s32 = nullptr;
+    ds->cand_s64 = nullptr;
+    ds->cand_cap = 0;
+    if (hipMalloc(&ds->cand, sizeof(unsigned long long) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (hipMalloc(&ds->cand_s32, sizeof(float) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (hipMalloc(&ds->cand_s64, sizeof(double) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
+    ds->cand_cap = cap;
+    return EPISDET_OK;
+}
+
+// Host tables.  FP64: exactly the oracle's (lgamma / log2 of glibc), so FP64
+// scores agree bit for bit.  FP32: the shifted K2 table or the MI table.
+int ensure_tables(episdet_dataset *ds, int score)
+{
+    ScoreTables &t = ds->tab[score];
+    if (t.ready) return EPISDET_OK;
+    const int N = ds->N[0] + ds->N[1];
+    const int len = N + 2;
+    std::vector<double> h64(len, 0.0);
+    std::vector<float> h32(len, 0.f);
+    if (score == EPISDET_SCORE_K2) {
+        const double c = std::log((double)N) - 1.0;
+        t.shift_c = c;
+        double mx = 0;
+        for (int m = 0; m < len; m++) {
+            h64[m] = std::lgamma((double)m + 1.0);
+            const double v = h64[m] - c * (double)m;
+            h32[m] = (float)v;
+            mx = std::max(mx, std::fabs(v));
+        }
+        t.t32_max = mx;
+    } else {
+        h64[0] = 0.0;
+        for (int m = 1; m <= N; m++) h64[m] = std::log2((double)m / (double)N);
+        for (int m = 0; m < len; m++) h32[m] = (float)h64[m];
+    }
+    if (hipMalloc(&t.t32, sizeof(float) * len) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (hipMalloc(&t.t64, sizeof(double) * len) != hipSuccess) return EPISDET_ERR_ALLOC;
+    HIPCHK(hipMemcpy(t.t32, h32.data(), sizeof(float) * len, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.t64, h64.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+    t.ready = true;
+    return EPISDET_OK;
+}
+
+double ulp32(double x)
+{
+    if (x <= 0) return std::ldexp(1.0, -149);
+    return std::ldexp(1.0, std::ilogb(x) - 23);
+}
+
+// FP32 scoring context and its error bound.  The bound covers the table
+// rounding (half an ulp of the largest entry per lookup), the two rounded
+// operations per cell, the compensated cell sum and the final subtraction;
+// the candidate margin is twice that.
+ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
+{
+    const ScoreTables &t = ds->tab[score];
+    const int ncells = order == 3 ? 27 : 9;
+    const int N = ds->N[0] + ds->N[1];
+    const double u = std::ldexp(1.0, -24);
+    ScoreCtx sc;
+    sc.score = score;
+    sc.ncells = ncells;
+    sc.t32 = t.t32;
+    sc.k2_total = 0.f;
+    sc.invN = (float)(1.0 / (double)N);
+    const double qc = (double)ds->N[0] / N, qt = (double)ds->N[1] / N;
+    sc.hY = (float)(-qc * std::log2(qc) - qt * std::log2(qt));
+    double eps;
+    if (score == EPISDET_SCORE_K2) {
+        sc.k2_total = (float)(ncells * t.shift_c);
+        const double T = t.t32_max;
+        const double sum_bound = ncells * 3.0 * T;
+        eps = ncells * 3.5 * ulp32(T) + 2.0 * u * sum_bound + ulp32(sum_bound) + ulp32(ncells * t.shift_c);
+    } else {
+        // Terms p*log2(p) are bounded by 0.531 in magnitude; the entropies by
+        // log2(2*ncells).  Each of the 3*ncells products carries a few
+        // roundings; the compensated sums add about one ulp of the result.
+        const double hmax = std::log2(2.0 * ncells);
+        eps = u * (3.0 * ncells * 0.531 * 6.0 + 8.0 * hmax) + 4.0 * u;
+    }
+    sc.margin = (float)(2.0 * eps);
+    return sc;
+}
+
+int ensure_pairs(episdet_dataset *ds);
+
+int launch_blocks(const void *kernel, int cu)
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, 256, 0) != hipSuccess || per < 1) per = 1;
+    return per * cu;
+}
+
+// ---------------------------------------------------------------- tiles --
+
+struct Tri3Tile {
+    int A, B;
+};
+constexpr Tri3Tile TRI_TILES[] = {{2, 4}, {2, 2}, {2, 4}, {4, 2}};   // id 0 = default
+constexpr int PAIR_TILES[] = {4, 2, 4};                               // R; id 0 = default
+
+template <int A, int B, int MODE>
+int launch_tri(episdet_dataset *ds, TriArgs &a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_triplets_valu<A, B, MODE>), dim3(grid), dim3(256), 0, s, a);
+    HIPCHK(hipGetLastError());
+    return EPISDET_OK;
+}
+
+template <int MODE>
+int dispatch_tri(episdet_dataset *ds, int tile, TriArgs &a, int grid_override, hipStream_t s)
+{
+    const Tri3Tile t = TRI_TILES[tile];
+    auto pick = [&](const void *k) { return grid_override > 0 ? grid_override : launch_blocks(k, ds->cu); };
+    if (t.A == 2 && t.B == 4) return launch_tri<2, 4, MODE>(ds, a, pick((const void *)k_triplets_valu<2, 4, MODE>), s);
+    if (t.A == 2 && t.B == 2) return launch_tri<2, 2, MODE>(ds, a, pick((const void *)k_triplets_valu<2, 2, MODE>), s);
+    if (t.A == 4 && t.B == 2) return launch_tri<4, 2, MODE>(ds, a, pick((const void *)k_triplets_valu<4, 2, MODE>), s);
+    return EPISDET_ERR_INVALID;
+}
+
+template <int MODE>
+int dispatch_pairs(episdet_dataset *ds, int tile, PairArgs &a, hipStream_t s)
+{
+    const int R = PAIR_TILES[tile];
+    if (R == 4) {
+        int grid = launch_blocks((const void *)k_pairs_valu<4, MODE>, ds->cu);
+        hipLaunchKernelGGL((k_pairs_valu<4, MODE>), dim3(grid), dim3(256), 0, s, a);
+    } else {
+        int grid = launch_blocks((const void *)k_pairs_valu<2, MODE>, ds->cu);
+        hipLaunchKernelGGL((k_pairs_valu<2, MODE>), dim3(grid), dim3(256), 0, s, a);
+    }
+    HIPCHK(hipGetLastError());
+    return EPISDET_OK;
+}
+
+// Groups of 3-way units: (Y tile, Z tile) pairs that hold at least one
+// X < Y < Z, each with the number of X values it needs.
+void build_groups(int n, int TY, int TZ, std::vector<int2> &groups, std::vector<unsigned long long> &gstart)
+{
+    groups.clear();
+    gstart.clear();
+    unsigned long long acc = 0;
+    const int nty = (n + TY - 1) / TY, ntz = (n + TZ - 1) / TZ;
+    for (int ty = 0; ty < nty; ty++) {
+        const int ylo = ty * TY, yhi = std::min(ylo + TY, n) - 1;
+        for (int tz = 0; tz < ntz; tz++) {
+            const int zhi = std::min(tz * TZ + TZ, n) - 1;
+            if (zhi <= ylo) continue;
+            const int xcount = std::min(yhi, zhi - 1);
+            if (xcount <= 0) continue;
+            groups.push_back(make_int2(ty, tz));
+            gstart.push_back(acc);
+            acc += (unsigned long long)xcount;
+        }
+    }
+    gstart.push_back(acc);
+}
+
+void fill_tri_common(episdet_dataset *ds, TriArgs &a)
+{
+    std::memset(&a, 0, sizeof a);
+    a.p0 = ds->planes[0];
+    a.p1 = ds->planes[1];
+    a.W0 = ds->W[0];
+    a.W1 = ds->W[1];
+    a.n = ds->n;
+    a.N0 = ds->N[0];
+    a.N1 = ds->N[1];
+    a.singles = ds->singles;
+    a.pairs = ds->pairs;
+    a.queue = &ds->ctl->queue;
+    a.checksum = &ds->ctl->checksum;
+    a.co.best_key = &ds->ctl->best32;
+    a.co.cand_count = &ds->ctl->cand_count;
+    a.co.cand = ds->cand;
+    a.co.cand_s32 = ds->cand_s32;
+    a.co.cand_cap = ds->cand_cap;
+}
+
+void fill_pair_common(episdet_dataset *ds, PairArgs &a, int T)
+{
+    std::memset(&a, 0, sizeof a);
+    a.p0 = ds->planes[0];
+    a.p1 = ds->planes[1];
+    a.W0 = ds->W[0];
+    a.W1 = ds->W[1];
+    a.n = ds->n;
+    a.N0 = ds->N[0];
+    a.N1 = ds->N[1];
+    a.singles = ds->singles;
+    a.ntiles = (ds->n + T - 1) / T;
+    a.nunits = (unsigned long long)a.ntiles * (a.ntiles + 1) / 2;
+    a.queue = &ds->ctl->queue;
+    a.checksum = &ds->ctl->checksum;
+    a.co.best_key = &ds->ctl->best32;
+    a.co.cand_count = &ds->ctl->cand_count;
+    a.co.cand = ds->cand;
+    a.co.cand_s32 = ds->cand_s32;
+    a.co.cand_cap = ds->cand_cap;
+}
+
+int init_ctl(episdet_dataset *ds)
+{
+    hipLaunchKernelGGL(k_init_search, dim3(1), dim3(64), 0, ds->stream, &ds->ctl->queue, &ds->ctl->best32,
+                       &ds->ctl->cand_count, &ds->ctl->best64, &ds->ctl->checksum);
+    HIPCHK(hipGetLastError());
+    return EPISDET_OK;
+}
+
+int ensure_pairs(episdet_dataset *ds)
+{
+    if (ds->pairs) return EPISDET_OK;
+    const size_t bytes = (size_t)ds->n * ds->n * 2 * sizeof(uint4);
+    if (hipMalloc(&ds->pairs, bytes) != hipSuccess) {
+        ds->pairs = nullptr;
+        return EPISDET_ERR_ALLOC;
+    }
+    CHK(init_ctl(ds));
+    PairArgs a;
+    fill_pair_common(ds, a, 16 * PAIR_TILES[0]);
+    a.pairs = ds->pairs;
+    CHK(dispatch_pairs<MODE_PAIRTABLE>(ds, 0, a, ds->stream));
+    HIPCHK(hipStreamSynchronize(ds->stream));
+    return EPISDET_OK;
+}
+
+inline bool better(int score, double s, unsigned long long p, double bs, unsigned long long bp)
+{
+    if (score == EPISDET_SCORE_K2) {
+        if (s < bs) return true;
+        if (s > bs) return false;
+    } else {
+        if (s > bs) return true;
+        if (s < bs) return false;
+    }
+    return p < bp;
+}
+
+void unpack(int order, unsigned long long p, int idx[3])
+{
+    if (order == 3) {
+        idx[0] = (int)(p & 0x1FFFFF);
+        idx[1] = (int)((p >> 21) & 0x1FFFFF);
+        idx[2] = (int)((p >> 42) & 0x1FFFFF);
+    } else {
+        idx[0] = (int)(p & 0xFFFFFFFFu);
+        idx[1] = (int)(p >> 32);
+        idx[2] = -1;
+    }
+}
+
+// FP64 re-scoring of `count` packed sets already on the device in ds->cand.
+int rescore(episdet_dataset *ds, int order, int score, int count, double *ms)
+{
+    if (count <= 0) {
+        *ms = 0;
+        return EPISDET_OK;
+    }
+    RescoreArgs r;
+    std::memset(&r, 0, sizeof r);
+    r.p0 = ds->planes[0];
+    r.p1 = ds->planes[1];
+    r.W0 = ds->W[0];
+    r.W1 = ds->W[1];
+    r.n = ds->n;
+    r.N0 = ds->N[0];
+    r.N1 = ds->N[1];
+    r.order = order;
+    r.score = score;
+    r.singles = ds->singles;
+    r.pairs = ds->pairs;
+    r.t64 = ds->tab[score].t64;
+    r.cand = ds->cand;
+    r.count = count;
+    r.out_s64 = ds->cand_s64;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ds->stream));
+    hipLaunchKernelGGL(k_rescore, dim3(count), dim3(64), 0, ds->stream, r);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, ds->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float f = 0;
+    HIPCHK(hipEventElapsedTime(&f, e0, e1));
+    *ms = f;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return EPISDET_OK;
+}
+
+int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res);
+
+// Fused search (VALU): one persistent launch, then FP64 re-scoring.
+int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
+{
+    const int order = o->order, score = o->score;
+    const int tile = o->tile;
+    if (order == 3 && (tile < 0 || tile >= (int)(sizeof TRI_TILES / sizeof TRI_TILES[0]))) return EPISDET_ERR_INVALID;
+    if (order == 2 && (tile < 0 || tile >= (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0]))) return EPISDET_ERR_INVALID;
+    CHK(ensure_tables(ds, score));
+    if (order == 3) CHK(ensure_pairs(ds));
+    CHK(ensure_cand(ds, o->candidate_cap > 0 ? (unsigned)o->candidate_cap : DEFAULT_CAP));
+    const ScoreCtx sc = make_ctx(ds, score, order);
+
+    unsigned long long *d_gstart = nullptr;
+    int2 *d_groups = nullptr;
+    TriArgs ta;
+    PairArgs pa;
+    if (order == 3) {
+        std::vector<int2> groups;
+        std::vector<unsigned long long> gstart;
+        const Tri3Tile t = TRI_TILES[tile];
+        build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
+        if (groups.empty()) return EPISDET_ERR_INVALID;
+        if (hipMalloc(&d_gstart, sizeof(unsigned long long) * gstart.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
+        if (hipMalloc(&d_groups, sizeof(int2) * groups.size()) != hipSuccess) {
+            (void)hipFree(d_gstart);
+            return EPISDET_ERR_ALLOC;
+        }
+        HIPCHK(hipMemcpy(d_gstart, gstart.data(), sizeof(unsigned long long) * gstart.size(), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_groups, groups.data(), sizeof(int2) * groups.size(), hipMemcpyHostToDevice));
+        fill_tri_common(ds, ta);
+        ta.sc = sc;
+        ta.gstart = d_gstart;
+        ta.groups = d_groups;
+        ta.ngroups = (int)groups.size();
+        ta.nunits = gstart.back();
+    } else {
+        fill_pair_common(ds, pa, 16 * PAIR_TILES[tile]);
+        pa.sc = sc;
+    }
+
+    CHK(init_ctl(ds));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, ds->stream));
+    int rc = order == 3 ? dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream)
+                        : dispatch_pairs<MODE_SEARCH>(ds, tile, pa, ds->stream);
+    HIPCHK(hipEventRecord(e1, ds->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    if (d_gstart) (void)hipFree(d_gstart);
+    if (d_groups) (void)hipFree(d_groups);
+    if (rc != EPISDET_OK) return rc;
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    res->kernel_ms = kms;
+    res->margin = sc.margin;
+    res->tile = tile;
+
+    Ctl hc;
+    HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
+    res->candidates = hc.cand_count;
+    if (hc.cand_count == 0) return EPISDET_ERR_INTERNAL;
+    if (hc.cand_count > ds->cand_cap) {
+        // Too many sets within the FP32 margin (heavy ties, degenerate data):
+        // answer exactly with the FP64 round loop instead (3-way).
+        if (order != 3) return EPISDET_ERR_CANDIDATES;
+        episdet_options po = *o;
+        po.path = EPISDET_PATH_PLAIN;
+        po.plain_rounds = 0;
+        const double k = res->kernel_ms;
+        CHK(run_plain(ds, &po, res));
+        res->kernel_ms += k;
+        res->fallback = 1;
+        res->path = EPISDET_PATH_FUSED_VALU;
+        return EPISDET_OK;
+    }
+    const int count = (int)hc.cand_count;
+    double rms = 0;
+    CHK(rescore(ds, order, score, count, &rms));
+    res->rescore_ms = rms;
+    std::vector<unsigned long long> hp(count);
+    std::vector<float> h32(count);
+    std::vector<double> h64(count);
+    HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h32.data(), ds->cand_s32, sizeof(float) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
+    double bs = score == EPISDET_SCORE_K2 ? INFINITY : -INFINITY;
+    unsigned long long bp = ~0ull;
+    float b32 = 0.f;
+    double maxabs = 0, maxrel = 0;
+    for (int i = 0; i < count; i++) {
+        const double s32 = score == EPISDET_SCORE_K2 ? (double)h32[i] : -(double)h32[i];
+        const double err = std::fabs(s32 - h64[i]);
+        maxabs = std::max(maxabs, err);
+        if (h64[i] != 0) maxrel = std::max(maxrel, err / std::fabs(h64[i]));
+        if (better(score, h64[i], hp[i], bs, bp)) {
+            bs = h64[i];
+            bp = hp[i];
+            b32 = (float)s32;
+        }
+    }
+    res->best_score = bs;
+    res->best_score_f32 = b32;
+    res->packed = bp;
+    unpack(order, bp, res->idx);
+    res->fp32_max_abs_err = maxabs;
+    res->fp32_max_rel_err = maxrel;
+    res->path = EPISDET_PATH_FUSED_VALU;
+    return EPISDET_OK;
+}
+
+// Reference-shaped 3-way path: host loop over (Y block, X, Z block) rounds
+// of 256 (search-triplets.cu:162-273) on NUM_STREAMS streams; each round is a
+// count kernel writing int32 tables to HBM and an FP64 objective kernel.
+// 2-way plain reuses the fused 2-way counting with FP64 candidates.
+int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
+{
+    const int order = o->order, score = o->score;
+    CHK(ensure_tables(ds, score));
+    CHK(ensure_cand(ds, o->candidate_cap > 0 ? (unsigned)o->candidate_cap : DEFAULT_CAP));
+    if (order == 2) {
+        // There is no separate 2-way round loop: PLAIN for pairs runs the
+        // fused 2-way kernel (its winner is exact after FP64 re-scoring).
+        episdet_options fo = *o;
+        fo.path = EPISDET_PATH_FUSED_VALU;
+        return run_fused(ds, &fo, res);
+    }
+    CHK(ensure_pairs(ds));
+    const int n = ds->n, nb = (n + BLOCK - 1) / BLOCK;
+    uint32_t *counts = nullptr;
+    const size_t slot = (size_t)2 * 256 * 256 * 8;
+    if (hipMalloc(&counts, sizeof(uint32_t) * slot * NUM_STREAMS) != hipSuccess) return EPISDET_ERR_ALLOC;
+    hipStream_t st[NUM_STREAMS];
+    for (int i = 0; i < NUM_STREAMS; i++) HIPCHK(hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking));
+    CHK(init_ctl(ds));
+    HIPCHK(hipStreamSynchronize(ds->stream));
+    hipEvent_t e0, e1, ev[NUM_STREAMS];
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    for (int i = 0; i < NUM_STREAMS; i++) HIPCHK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+    HIPCHK(hipEventRecord(e0, ds->stream));
+    for (int i = 0; i < NUM_STREAMS; i++) HIPCHK(hipStreamWaitEvent(st[i], e0, 0));
+
+    const int plain_tile = 1;   // 32 x 32 (Y, Z) per workgroup: 64 workgroups per round
+    TriArgs ta;
+    fill_tri_common(ds, ta);
+    PlainObjArgs po;
+    std::memset(&po, 0, sizeof po);
+    po.singles = ds->singles;
+    po.pairs = ds->pairs;
+    po.n = n;
+    po.N0 = ds->N[0];
+    po.N1 = ds->N[1];
+    po.score = score;
+    po.t64 = ds->tab[score].t64;
+    po.best_key = &ds->ctl->best64;
+    po.cand = ds->cand;
+    po.cand_s64 = ds->cand_s64;
+    po.cand_count = &ds->ctl->cand_count;
+    po.cand_cap = ds->cand_cap;
+    const Tri3Tile t = TRI_TILES[plain_tile];
+    const int grid_count = (256 / (16 * t.A)) * (256 / (16 * t.B));
+    long long rounds = 0;
+    const long long limit = o->plain_rounds > 0 ? o->plain_rounds : -1;
+    bool stop = false;
+    for (int yb = 0; yb < nb && !stop; yb++) {
+        for (int X = 0; X < std::min((yb + 1) * BLOCK, n) && !stop; X++) {
+            for (int zb = yb; zb < nb; zb++) {
+                if (limit >= 0 && rounds >= limit) {
+                    stop = true;
+                    break;
+                }
+                const int s = (int)(rounds % NUM_STREAMS);
+                uint32_t *c = counts + slot * s;
+                ta.X = X;
+                ta.yb = yb;
+                ta.zb = zb;
+                ta.counts = c;
+                CHK(dispatch_tri<MODE_COUNTS>(ds, plain_tile, ta, grid_count, st[s]));
+                po.counts = c;
+                po.X = X;
+                po.yb = yb;
+                po.zb = zb;
+                hipLaunchKernelGGL(k_triplets_objective_plain, dim3(256), dim3(256), 0, st[s], po);
+                HIPCHK(hipGetLastError());
+                rounds++;
+            }
+        }
+    }
+    for (int i = 0; i < NUM_STREAMS; i++) {
+        HIPCHK(hipEventRecord(ev[i], st[i]));
+        HIPCHK(hipStreamWaitEvent(ds->stream, ev[i], 0));
+    }
+    HIPCHK(hipEventRecord(e1, ds->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
+    res->kernel_ms = kms;
+    res->rounds_run = rounds;
+    for (int i = 0; i < NUM_STREAMS; i++) {
+        (void)hipEventDestroy(ev[i]);
+        (void)hipStreamDestroy(st[i]);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(counts);
+
+    Ctl hc;
+    HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
+    res->candidates = hc.cand_count;
+    if (hc.cand_count > ds->cand_cap) return EPISDET_ERR_CANDIDATES;
+    const int count = (int)hc.cand_count;
+    std::vector<unsigned long long> hp(count);
+    std::vector<double> h64(count);
+    if (count) {
+        HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
+    }
+    double bb = INFINITY;
+    unsigned long long bp = ~0ull;
+    for (int i = 0; i < count; i++) {
+        if (h64[i] < bb || (h64[i] == bb && hp[i] < bp)) {
+            bb = h64[i];
+            bp = hp[i];
+        }
+    }
+    res->best_score = score == EPISDET_SCORE_K2 ? bb : -bb;
+    res->best_score_f32 = (float)res->best_score;
+    res->packed = bp;
+    unpack(order, bp, res->idx);
+    res->path = EPISDET_PATH_PLAIN;
+    return EPISDET_OK;
+}
+
+long long sets_total(int order, int n) { return (long long)episdet_n_choose_k((unsigned)n, (unsigned)order); }
+
+}  // namespace
+
+// ================================================================ C ABI ==
+
+extern "C" {
+
+unsigned long long episdet_n_choose_k(unsigned n, unsigned k)
+{
+    unsigned long long r = 1;
+    for (unsigned i = 1; i <= k; i++) {
+        r = r * n / i;
+        n = n - 1;
+    }
+    return r;
+}
+
+long long episdet_reference_rounds(int order, int numSNPs)
+{
+    const long long nb = (numSNPs + BLOCK - 1) / BLOCK;
+    if (order == 2) return nb * (nb + 1) / 2;
+    long long r = 0;
+    for (long long yb = 0; yb < nb; yb++) r += std::min((yb + 1) * BLOCK, (long long)numSNPs) * (nb - yb);
+    return r;
+}
+
+const char *episdet_status_string(int status)
+{
+    switch (status) {
+    case EPISDET_OK: return "ok";
+    case EPISDET_ERR_INVALID: return "invalid argument";
+    case EPISDET_ERR_NO_DEVICE: return "no gfx950 device";
+    case EPISDET_ERR_HIP: return "HIP runtime error";
+    case EPISDET_ERR_ALLOC: return "allocation failed";
+    case EPISDET_ERR_CANDIDATES: return "candidate buffer overflow";
+    default: return "internal error";
+    }
+}
+
+int episdet_device_info(char *name, int name_len, int *compute_units, int *clock_khz)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count < 1) return EPISDET_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, 0));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (compute_units) *compute_units = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    return EPISDET_OK;
+}
+
+static int compute_singles(episdet_dataset *ds)
+{
+    hipLaunchKernelGGL(k_singles, dim3(ds->n), dim3(64), 0, ds->stream, ds->planes[0], ds->W[0], ds->planes[1],
+                       ds->W[1], ds->n, ds->singles);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ds->stream));
+    return EPISDET_OK;
+}
+
+int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int numSNPs, int numCases,
+                           int numControls, int casesWords, int controlsWords, episdet_dataset **out)
+{
+    if (!out) return EPISDET_ERR_INVALID;
+    *out = nullptr;
+    if (!cases || !controls || numSNPs < 2 || numSNPs > 0x1FFFFF || numCases < 1 || numControls < 1)
+        return EPISDET_ERR_INVALID;
+    if (casesWords * 32LL < numCases || controlsWords * 32LL < numControls) return EPISDET_ERR_INVALID;
+    int cu = 0;
+    CHK(check_device(&cu));
+    HIPCHK(hipSetDevice(0));
+    episdet_dataset *ds = new (std::nothrow) episdet_dataset();
+    if (!ds) return EPISDET_ERR_ALLOC;
+    ds->cu = cu;
+    ds->n = numSNPs;
+    ds->nrows = ((numSNPs + BLOCK - 1) / BLOCK) * BLOCK;
+    ds->N[0] = numCases;
+    ds->N[1] = numControls;
+    const int in_w[2] = {casesWords, controlsWords};
+    const unsigned *src[2] = {cases, controls};
+    int rc = EPISDET_OK;
+    if (hipStreamCreateWithFlags(&ds->stream, hipStreamNonBlocking) != hipSuccess) rc = EPISDET_ERR_HIP;
+    if (rc == EPISDET_OK && hipMalloc(&ds->ctl, sizeof(Ctl)) != hipSuccess) rc = EPISDET_ERR_ALLOC;
+    for (int c = 0; c < 2 && rc == EPISDET_OK; c++) {
+        const int Wv = (ds->N[c] + 31) / 32;
+        ds->W[c] = ((Wv + KW - 1) / KW) * KW;
+        const size_t in_bytes = sizeof(uint32_t) * (size_t)numSNPs * 2 * in_w[c];
+        const size_t dev_words = (size_t)ds->nrows * 2 * ds->W[c];
+        uint32_t *stage = nullptr;
+        if (hipMalloc(&ds->planes[c], sizeof(uint32_t) * dev_words) != hipSuccess ||
+            hipMalloc(&stage, in_bytes) != hipSuccess) {
+            if (stage) (void)hipFree(stage);
+            rc = EPISDET_ERR_ALLOC;
+            break;
+        }
+        if (hipMemcpyAsync(stage, src[c], in_bytes, hipMemcpyHostToDevice, ds->stream) != hipSuccess) rc = EPISDET_ERR_HIP;
+        if (rc == EPISDET_OK) {
+            const int grid = (int)std::min<size_t>((dev_words + 255) / 256, 65536);
+            hipLaunchKernelGGL(k_repack, dim3(grid), dim3(256), 0, ds->stream, stage, in_w[c], ds->planes[c], ds->W[c],
+                               numSNPs, ds->nrows, ds->N[c]);
+            if (hipGetLastError() != hipSuccess) rc = EPISDET_ERR_HIP;
+        }
+        if (hipStreamSynchronize(ds->stream) != hipSuccess) rc = EPISDET_ERR_HIP;
+        (void)hipFree(stage);
+    }
+    if (rc == EPISDET_OK && hipMalloc(&ds->singles, sizeof(uint4) * numSNPs) != hipSuccess) rc = EPISDET_ERR_ALLOC;
+    if (rc == EPISDET_OK) rc = compute_singles(ds);
+    if (rc != EPISDET_OK) {
+        episdet_dataset_destroy(ds);
+        return rc;
+    }
+    *out = ds;
+    return EPISDET_OK;
+}
+
+void episdet_dataset_destroy(episdet_dataset *ds)
+{
+    if (!ds) return;
+    (void)hipDeviceSynchronize();
+    for (int c = 0; c < 2; c++) {
+        if (ds->planes[c]) (void)hipFree(ds->planes[c]);
+        if (ds->tab[c].t32) (void)hipFree(ds->tab[c].t32);
+        if (ds->tab[c].t64) (void)hipFree(ds->tab[c].t64);
+    }
+    if (ds->singles) (void)hipFree(ds->singles);
+    if (ds->pairs) (void)hipFree(ds->pairs);
+    if (ds->ctl) (void)hipFree(ds->ctl);
+    if (ds->cand) (void)hipFree(ds->cand);
+    if (ds->cand_s32) (void)hipFree(ds->cand_s32);
+    if (ds->cand_s64) (void)hipFree(ds->cand_s64);
+    if (ds->stream) (void)hipStreamDestroy(ds->stream);
+    delete ds;
+}
+
+int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result *res)
+{
+    if (!ds || !opt || !res) return EPISDET_ERR_INVALID;
+    if (opt->order != 2 && opt->order != 3) return EPISDET_ERR_INVALID;
+    if (opt->score != EPISDET_SCORE_K2 && opt->score != EPISDET_SCORE_MI) return EPISDET_ERR_INVALID;
+    if (ds->n < opt->order) return EPISDET_ERR_INVALID;
+    std::memset(res, 0, sizeof *res);
+    const double t0 = now_s();
+    if (opt->rebuild) {
+        CHK(compute_singles(ds));
+        if (ds->pairs) {
+            (void)hipFree(ds->pairs);
+            ds->pairs = nullptr;
+        }
+        if (opt->order == 3) CHK(ensure_pairs(ds));
+        res->prep_ms = (now_s() - t0) * 1e3;
+    }
+    int rc;
+    switch (opt->path) {
+    case EPISDET_PATH_AUTO:
+    case EPISDET_PATH_FUSED_VALU: rc = run_fused(ds, opt, res); break;
+    case EPISDET_PATH_PLAIN: rc = run_plain(ds, opt, res); break;
+    default: rc = EPISDET_ERR_INVALID;
+    }
+    res->wall_s = now_s() - t0;
+    res->rounds = episdet_reference_rounds(opt->order, ds->n);
+    res->sets = sets_total(opt->order, ds->n);
+    if (opt->path == EPISDET_PATH_PLAIN && opt->order == 3 && res->rounds_run < res->rounds && res->rounds > 0)
+        res->sets = (long long)((double)res->sets * (double)res->rounds_run / (double)res->rounds);
+    return rc;
+}
+
+int episdet_search(int order, int score, const unsigned *cases, const unsigned *controls, int numSNPs, int numCases,
+                   int numControls, unsigned numSNPsWithPadding, int numCasesWithPadding, int numControlsWithPadding,
+                   int *roundsCounter, double *searchTime, float *outputFromGpu, unsigned long long *packedIndices)
+{
+    if (!roundsCounter || !searchTime || !outputFromGpu || !packedIndices) return EPISDET_ERR_INVALID;
+    if (numCasesWithPadding % 32 != 0 || numControlsWithPadding % 32 != 0) return EPISDET_ERR_INVALID;
+    if ((long long)numSNPsWithPadding < numSNPs) return EPISDET_ERR_INVALID;
+    const double t0 = now_s();
+    episdet_dataset *ds = nullptr;
+    CHK(episdet_dataset_create(cases, controls, numSNPs, numCases, numControls, numCasesWithPadding / 32,
+                               numControlsWithPadding / 32, &ds));
+    episdet_options o;
+    std::memset(&o, 0, sizeof o);
+    o.order = order;
+    o.score = score;
+    o.path = EPISDET_PATH_AUTO;
+    episdet_result r;
+    const int rc = episdet_run(ds, &o, &r);
+    episdet_dataset_destroy(ds);
+    *searchTime = now_s() - t0;
+    if (rc != EPISDET_OK) return rc;
+    *roundsCounter = (int)episdet_reference_rounds(order, numSNPs);
+    *outputFromGpu = (float)r.best_score;
+    *packedIndices = r.packed;
+    return EPISDET_OK;
+}
+
+int episdet_tables(episdet_dataset *ds, int order, const int *sets, int m, unsigned *out)
+{
+    if (!ds || !sets || !out || m < 0 || (order != 2 && order != 3)) return EPISDET_ERR_INVALID;
+    if (m == 0) return EPISDET_OK;
+    for (int i = 0; i < m; i++) {
+        for (int o = 0; o < order; o++) {
+            const int v = sets[i * order + o];
+            if (v < 0 || v >= ds->n || (o > 0 && v <= sets[i * order + o - 1])) return EPISDET_ERR_INVALID;
+        }
+    }
+    if (order == 3) CHK(ensure_pairs(ds));
+    std::vector<unsigned long long> hp(m);
+    for (int i = 0; i < m; i++) {
+        const int *s = sets + (size_t)i * order;
+        hp[i] = order == 3 ? ((unsigned long long)s[0] | ((unsigned long long)s[1] << 21) |
+                              ((unsigned long long)s[2] << 42))
+                           : ((unsigned long long)s[0] | ((unsigned long long)s[1] << 32));
+    }
+    const int nc = order == 3 ? 27 : 9;
+    unsigned long long *d_sets = nullptr;
+    uint32_t *d_cells = nullptr;
+    if (hipMalloc(&d_sets, sizeof(unsigned long long) * m) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (hipMalloc(&d_cells, sizeof(uint32_t) * (size_t)m * 2 * nc) != hipSuccess) {
+        (void)hipFree(d_sets);
+        return EPISDET_ERR_ALLOC;
+    }
+    HIPCHK(hipMemcpy(d_sets, hp.data(), sizeof(unsigned long long) * m, hipMemcpyHostToDevice));
+    RescoreArgs r;
+    std::memset(&r, 0, sizeof r);
+    r.p0 = ds->planes[0];
+    r.p1 = ds->planes[1];
+    r.W0 = ds->W[0];
+    r.W1 = ds->W[1];
+    r.n = ds->n;
+    r.N0 = ds->N[0];
+    r.N1 = ds->N[1];
+    r.order = order;
+    r.singles = ds->singles;
+    r.pairs = ds->pairs;
+    r.cand = d_sets;
+    r.count = m;
+    r.out_cells = d_cells;
+    hipLaunchKernelGGL(k_rescore, dim3(m), dim3(64), 0, ds->stream, r);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(ds->stream));
+    HIPCHK(hipMemcpy(out, d_cells, sizeof(uint32_t) * (size_t)m * 2 * nc, hipMemcpyDeviceToHost));
+    (void)hipFree(d_sets);
+    (void)hipFree(d_cells);
+    return EPISDET_OK;
+}
+
+static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *out, size_t capacity,
+                            unsigned long long *checksum)
+{
+    if (!ds || (order != 2 && order != 3)) return EPISDET_ERR_INVALID;
+    if (path != EPISDET_PATH_FUSED_VALU && path != EPISDET_PATH_AUTO) return EPISDET_ERR_INVALID;
+    const int nc = order == 3 ? 27 : 9;
+    const unsigned long long total = episdet_n_choose_k((unsigned)ds->n, (unsigned)order);
+    uint32_t *d_dump = nullptr;
+    if (out) {
+        if (capacity < total * 2 * nc) return EPISDET_ERR_INVALID;
+        if (hipMalloc(&d_dump, sizeof(uint32_t) * total * 2 * nc) != hipSuccess) return EPISDET_ERR_ALLOC;
+    }
+    if (order == 3) CHK(ensure_pairs(ds));
+    CHK(init_ctl(ds));
+    int rc;
+    unsigned long long *d_gstart = nullptr;
+    int2 *d_groups = nullptr;
+    if (order == 3) {
+        std::vector<int2> groups;
+        std::vector<unsigned long long> gstart;
+        const Tri3Tile t = TRI_TILES[0];
+        build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
+        if (hipMalloc(&d_gstart, sizeof(unsigned long long) * gstart.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
+        if (hipMalloc(&d_groups, sizeof(int2) * std::max<size_t>(groups.size(), 1)) != hipSuccess) return EPISDET_ERR_ALLOC;
+        HIPCHK(hipMemcpy(d_gstart, gstart.data(), sizeof(unsigned long long) * gstart.size(), hipMemcpyHostToDevice));
+        if (!groups.empty())
+            HIPCHK(hipMemcpy(d_groups, groups.data(), sizeof(int2) * groups.size(), hipMemcpyHostToDevice));
+        TriArgs ta;
+        fill_tri_common(ds, ta);
+        ta.gstart = d_gstart;
+        ta.groups = d_groups;
+        ta.ngroups = (int)groups.size();
+        ta.nunits = gstart.back();
+        ta.dump = d_dump;
+        rc = out ? dispatch_tri<MODE_DUMP>(ds, 0, ta, 0, ds->stream) : dispatch_tri<MODE_CHECKSUM>(ds, 0, ta, 0, ds->stream);
+    } else {
+        PairArgs pa;
+        fill_pair_common(ds, pa, 16 * PAIR_TILES[0]);
+        pa.dump = d_dump;
+        rc = out ? dispatch_pairs<MODE_DUMP>(ds, 0, pa, ds->stream) : dispatch_pairs<MODE_CHECKSUM>(ds, 0, pa, ds->stream);
+    }
+    HIPCHK(hipStreamSynchronize(ds->stream));
+    if (d_gstart) (void)hipFree(d_gstart);
+    if (d_groups) (void)hipFree(d_groups);
+    if (rc == EPISDET_OK && out)
+        HIPCHK(hipMemcpy(out, d_dump, sizeof(uint32_t) * total * 2 * nc, hipMemcpyDeviceToHost));
+    if (rc == EPISDET_OK && checksum) {
+        Ctl hc;
+        HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
+        *checksum = hc.checksum;
+    }
+    if (d_dump) (void)hipFree(d_dump);
+    return rc;
+}
+
+int episdet_dump_tables(episdet_dataset *ds, int order, int path, unsigned *out, size_t capacity)
+{
+    if (!out) return EPISDET_ERR_INVALID;
+    return dump_or_checksum(ds, order, path, out, capacity, nullptr);
+}
+
+int episdet_checksum(episdet_dataset *ds, int order, int path, unsigned long long *checksum)
+{
+    if (!checksum) return EPISDET_ERR_INVALID;
+    return dump_or_checksum(ds, order, path, nullptr, 0, checksum);
+}
+
+}  // extern "C"

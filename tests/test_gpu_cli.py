@@ -1,0 +1,54 @@
+"""The drop-in CLI binaries on the GPU: stdout lines of the reference's main
+(tensor-episdet.cu:99-102,150,164,181-201) and the same winner as the oracle."""
+import os
+import re
+import subprocess
+
+import pytest
+
+import episdet
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("order,score", [(3, "k2"), (3, "mi"), (2, "k2"), (2, "mi")])
+def test_cli_output(tmp_path, order, score):
+    syn = episdet.generate(300, 1500, 1400, seed=31, planted_sets=1, planted_order=order)
+    txt = episdet.write_dataset(tmp_path, "db", syn)
+    exe = os.path.join(episdet.BIN_DIR, f"tensor-episdet.{'triplets' if order == 3 else 'pairs'}.{score}.bin")
+    print("cli", exe, flush=True)
+    p = subprocess.run([exe, txt], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    lines = p.stdout.splitlines()
+    assert lines[0] == "Num. SNPs: 300"
+    assert lines[1] == "Num. Blocks of SNPs: 2"
+    assert lines[2] == "Num. Cases: 1500"
+    assert lines[3] == "Num. Controls: 1400"
+    assert lines[4] == "-" * 31
+    assert lines[5:7] == ["Outer loop iteration 0 out of 2", "Outer loop iteration 1 out of 2"]
+    assert lines[7] == "-" * 31
+    if order == 3:
+        m = re.fullmatch(r"\{SNP_X_i, SNP_Y_i, SNP_Z_i\}: SCORE\t->\t\{(\d+), (\d+), (\d+)\}: (-?\d+\.\d{6})", lines[8])
+    else:
+        m = re.fullmatch(r"\{SNP_X_i, SNP_Y_i\}: SCORE\t->\t\{(\d+), (\d+)\}: (-?\d+\.\d{6})", lines[8])
+    assert m, lines[8]
+    idx = tuple(int(v) for v in m.groups()[:order])
+    o = O.search(syn.cases, syn.controls, 1500, 1400, order, O.K2 if score == "k2" else O.MI)
+    assert idx == tuple(o["idx"])
+    assert float(m.groups()[-1]) == pytest.approx(o["score"], rel=1e-6)
+    rounds = episdet.reference_rounds(order, 300)
+    assert lines[9] == f"Num. of rounds processed: {rounds}"
+    assert re.fullmatch(r"Wall-clock time:\t\d+\.\d{3} seconds", lines[10])
+    assert re.fullmatch(r"Tensor TOPS: \d+\.\d{3}", lines[11])
+    assert re.fullmatch(r"Num\. unique sets per sec\. \(scaled to sample size\): \d+\.\d{3} × 10\^12", lines[12])
+    assert lines[13] == f"Unique sets of SNPs evaluated (k={order}): {episdet.n_choose_k(300, order)}"
+    assert re.fullmatch(r"Ratio of unique sets of SNPs: \d+\.\d{3}", lines[14])
+
+
+def test_cli_errors(tmp_path):
+    exe = os.path.join(episdet.BIN_DIR, "tensor-episdet.triplets.k2.bin")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1 and "Usage" in p.stderr
+    p = subprocess.run([exe, str(tmp_path / "nope.txt")], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 1 and "does not exist" in p.stderr

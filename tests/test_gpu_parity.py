@@ -1,0 +1,213 @@
+"""GPU parity: the gfx950 kernels against the CPU oracle and the golden
+fixtures, through the C ABI.
+
+Bar: every contingency table bit-exact; the winning set identical (ties to
+the lowest packed index); the winner's FP64 score bit-identical to the
+oracle's FP64 score (same tables, same operation sequence); the FP32 scores
+of the fused kernel within 1e-5 relative of FP64 for K2.
+"""
+import itertools
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import episdet
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+K2_FP32_REL_TOL = 1e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def packed_of(order, s):
+    return s[0] | (s[1] << 21) | (s[2] << 42) if order == 3 else s[0] | (s[1] << 32)
+
+
+def check_best(ds, syn_cases, syn_controls, Nc, Nt, order, score, path=episdet.PATH_FUSED_VALU, tile=0):
+    say(f"  best: order={order} score={score} path={path} tile={tile} n={syn_cases.shape[0]} Nc={Nc} Nt={Nt}")
+    r = ds.run(order, score, path=path, tile=tile)
+    o = O.search(syn_cases, syn_controls, Nc, Nt, order, score)
+    assert r["packed"] == o["packed"], (r, o)
+    assert r["best_score"] == o["score"], (r["best_score"], o["score"])
+    return r
+
+
+# ------------------------------------------------------------- golden ----
+
+def test_golden_tables_and_best(golden, golden_planes):
+    cases, controls, Nc, Nt = golden_planes
+    with episdet.Dataset(cases, controls, Nc, Nt) as ds:
+        for order in (2, 3):
+            say(f"golden dump order={order}")
+            dump = ds.dump_tables(order)
+            sets = golden[f"order{order}"]["sets"]
+            assert dump.shape[0] == len(sets)
+            for s in sets:
+                snps = s["snps"]
+                r = sum(math.comb(v, i + 1) for i, v in enumerate(snps))
+                assert list(dump[r, 0]) == s["cases"] and list(dump[r, 1]) == s["controls"], snps
+            t = ds.tables(order, [s["snps"] for s in sets])
+            assert (t[:, 0] == np.array([s["cases"] for s in sets])).all()
+            assert (t[:, 1] == np.array([s["controls"] for s in sets])).all()
+            for score, key, sign in ((episdet.K2, "k2", 1), (episdet.MI, "mi", -1)):
+                r = ds.run(order, score)
+                best = min(sign * s[key] for s in sets)
+                tied = [s["packed"] for s in sets if abs(sign * s[key] - best) <= 1e-9 * max(1.0, abs(best))]
+                assert r["packed"] == min(tied)
+                assert sign * r["best_score"] == pytest.approx(best, rel=1e-12, abs=1e-12)
+
+
+# -------------------------------------------------- random vs oracle ----
+
+CASES = [
+    # n, Nc, Nt, words_c, words_t, seed
+    (24, 45, 70, 0, 0, 1),
+    (67, 1000, 1100, 0, 0, 2),
+    (130, 2048, 2048, 0, 0, 3),
+    (90, 3001, 2500, 101, 79, 4),       # arbitrary padding (W % 128 != 0)
+    (33, 1, 1, 0, 0, 5),                # one sample per class
+]
+
+
+@pytest.mark.parametrize("n,Nc,Nt,wc,wt,seed", CASES)
+def test_fused_tables_bit_exact(n, Nc, Nt, wc, wt, seed):
+    syn = episdet.generate(n, Nc, Nt, seed=seed, cases_words=wc, controls_words=wt)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for order in (2, 3):
+            say(f"dump order={order} n={n} Nc={Nc} Nt={Nt}")
+            dump = ds.dump_tables(order)
+            ref, cs = O.all_tables(syn.cases, syn.controls, Nc, Nt, order)
+            assert (dump == ref).all()
+            say(f"checksum order={order}")
+            assert ds.checksum(order) == cs
+
+
+@pytest.mark.parametrize("n,Nc,Nt,wc,wt,seed", CASES)
+def test_fused_best_matches_oracle(n, Nc, Nt, wc, wt, seed):
+    syn = episdet.generate(n, Nc, Nt, seed=seed, cases_words=wc, controls_words=wt,
+                           planted_sets=1 if n >= 60 else 0, planted_order=3)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for order in (2, 3):
+            for score in (episdet.K2, episdet.MI):
+                r = check_best(ds, syn.cases, syn.controls, Nc, Nt, order, score)
+                if score == episdet.K2 and r["fallback"] == 0:
+                    assert r["fp32_max_rel_err"] <= K2_FP32_REL_TOL
+
+
+@pytest.mark.parametrize("tile", [0, 1, 3])
+def test_triplet_tiles_agree(tile):
+    syn = episdet.generate(150, 1500, 1700, seed=11, planted_sets=1)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        check_best(ds, syn.cases, syn.controls, 1500, 1700, 3, episdet.K2, tile=tile)
+
+
+def test_plain_path_matches_oracle():
+    syn = episdet.generate(300, 1024, 1024, seed=12, planted_sets=1)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for score in (episdet.K2, episdet.MI):
+            r = check_best(ds, syn.cases, syn.controls, 1024, 1024, 3, score, path=episdet.PATH_PLAIN)
+            assert r["rounds_run"] == episdet.reference_rounds(3, 300)
+        assert tuple(r["idx"]) == syn.planted[0]
+
+
+def test_explicit_set_tables():
+    syn = episdet.generate(200, 5000, 7000, seed=13)
+    rng = np.random.default_rng(0)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for order in (2, 3):
+            sets = np.array([np.sort(rng.choice(200, order, replace=False)) for _ in range(300)], dtype=np.int32)
+            say(f"tables order={order}")
+            t = ds.tables(order, sets)
+            for i, s in enumerate(sets):
+                assert (t[i, 0] == O.table_bits(syn.cases, 5000, s)).all()
+                assert (t[i, 1] == O.table_bits(syn.controls, 7000, s)).all()
+
+
+# -------------------------------------------------------- edge cases ----
+
+def test_minimum_sizes():
+    for n, order in ((2, 2), (3, 3), (3, 2)):
+        syn = episdet.generate(n, 100, 90, seed=n)
+        with episdet.Dataset.from_synthetic(syn) as ds:
+            check_best(ds, syn.cases, syn.controls, 100, 90, order, episdet.K2)
+
+
+def test_degenerate_data_all_homozygous_and_duplicates():
+    # every SNP monomorphic (all genotype 0): every table has one full cell,
+    # all scores tie exactly, the lowest packed index must win.
+    n, Nc, Nt = 40, 3000, 2000
+    cases = np.zeros((n, 2, episdet.padded_words(Nc)), dtype=np.uint32)
+    controls = np.zeros((n, 2, episdet.padded_words(Nt)), dtype=np.uint32)
+    for arr, N in ((cases, Nc), (controls, Nt)):
+        full, rem = divmod(N, 32)
+        arr[:, 0, :full] = 0xFFFFFFFF
+        if rem:
+            arr[:, 0, full] = (0xFFFFFFFF << (32 - rem)) & 0xFFFFFFFF
+    with episdet.Dataset(cases, controls, Nc, Nt) as ds:
+        for order in (2, 3):
+            dump = ds.dump_tables(order)
+            ref, _ = O.all_tables(cases, controls, Nc, Nt, order)
+            assert (dump == ref).all()
+            for score in (episdet.K2, episdet.MI):
+                r = check_best(ds, cases, controls, Nc, Nt, order, score)
+                assert r["packed"] == packed_of(order, tuple(range(order)))
+
+
+def test_duplicate_snps_tie_rule():
+    syn = episdet.generate(80, 2000, 2000, seed=21)
+    syn.cases[50], syn.controls[50] = syn.cases[10], syn.controls[10]
+    syn.cases[30] = syn.cases[10]       # strong pairs (10,30) and its twin (30,50)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for order in (2, 3):
+            for score in (episdet.K2, episdet.MI):
+                check_best(ds, syn.cases, syn.controls, 2000, 2000, order, score)
+
+
+# --------------------------------------------- reference shapes (small) --
+
+def test_config1_all_pair_tables():
+    """512 SNPs, 1024 cases + 1024 controls: all 130,816 pair tables
+    bit-exact and the best pair identical."""
+    syn = episdet.generate(512, 1024, 1024, seed=1)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        say("config1 dump")
+        dump = ds.dump_tables(2)
+        ref, cs = O.all_tables(syn.cases, syn.controls, 1024, 1024, 2)
+        assert dump.shape[0] == 130_816
+        assert (dump == ref).all()
+        assert ds.checksum(2) == cs
+        for score in (episdet.K2, episdet.MI):
+            check_best(ds, syn.cases, syn.controls, 1024, 1024, 2, score)
+
+
+def test_full_sample_axis_planted_triplet():
+    """The benchmark's sample axis (131072 + 131072) on 192 SNPs: winner
+    identical to the oracle, the planted triplet recovered, FP32 within tol."""
+    syn = episdet.generate(192, 131072, 131072, seed=2, planted_sets=1, planted_order=3)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        r = check_best(ds, syn.cases, syn.controls, 131072, 131072, 3, episdet.K2)
+        assert tuple(r["idx"]) == syn.planted[0]
+        assert r["fp32_max_rel_err"] <= K2_FP32_REL_TOL
+        say("checksum at full sample axis vs plain tables")
+        sets = np.array(list(itertools.islice(itertools.combinations(range(192), 3), 0, 4000, 7)), dtype=np.int32)
+        t = ds.tables(3, sets)
+        for i in range(0, len(sets), 50):
+            assert (t[i, 0] == O.table_bits(syn.cases, 131072, sets[i])).all()
+            assert (t[i, 1] == O.table_bits(syn.controls, 131072, sets[i])).all()
+
+
+def test_mi_pairs_uneven_classes_planted():
+    """Config-3 style: 40/60 classes with padding that is not a 1024-bit
+    multiple of the valid bits, MAF ~ Beta(0.5, 2), planted pair."""
+    syn = episdet.generate(1024, 26214, 39322, seed=3, maf_dist=episdet.MAF_BETA, maf=(0.01, 0.5),
+                           planted_sets=1, planted_order=2)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        r = check_best(ds, syn.cases, syn.controls, 26214, 39322, 2, episdet.MI)
+        assert tuple(r["idx"]) == syn.planted[0]
