@@ -210,4 +210,4 @@ def test_mi_pairs_uneven_classes_planted():
                            planted_sets=1, planted_order=2)
     with episdet.Dataset.from_synthetic(syn) as ds:
         r = check_best(ds, syn.cases, syn.controls, 26214, 39322, 2, episdet.MI)
-        assert tuple(r["idx"]) == syn.planted[0]
+        assert tuple(r["idx"][:2]) == syn.planted[0]
