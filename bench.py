@@ -110,15 +110,17 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         threads = min(os.cpu_count() or 1, 16)
-        # calibrate a slice of the search (largest index in [z0, z1)) to about
-        # --cpu-seconds of work
+        # calibrate a slice of the search (largest index in [z0, z0+zn)) to
+        # about --cpu-seconds: the marginal cost of one more Z from two probes
+        # (each probe also pays for its pair table, which cancels out)
         z0 = max(order, min(n // 4, 600))
-        probe = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + 2))
-        t0 = time.perf_counter()
-        probe = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + 2))
-        rate = probe["sets"] / max(time.perf_counter() - t0, 1e-6)
-        per_z = probe["sets"] / 2
-        zn = int(max(2, min(n - z0, args.cpu_seconds * rate / per_z)))
+        probe_t = []
+        for k in (4, 12):
+            t0 = time.perf_counter()
+            oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + k))
+            probe_t.append(time.perf_counter() - t0)
+        per_z = max((probe_t[1] - probe_t[0]) / 8, 1e-4)
+        zn = int(max(4, min(n - z0, args.cpu_seconds / per_z)))
         t0 = time.perf_counter()
         c = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + zn))
         dt = time.perf_counter() - t0
