@@ -32,7 +32,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tensor-episdet_amd"))
 
 VALU_PEAK_OPS = 256 * 64 * 2.4e9        # 32-bit VALU lane-ops/s, MI355X
-OPS_PER_SET_SAMPLE = {2: 0.25, 3: 0.5}
+OPS_PER_SET_SAMPLE = {2: 0.25, 3: 0.5}   # VALU: AND + BCNT per cell per 32 samples
+MFMA_FP4_PEAK = 10.1e15                  # dense FP4 MFMA flop/s, MI355X
+FLOPS_PER_SET_SAMPLE = {2: 8.0, 3: 16.0} # MFMA: 2^k multiply-adds per set-sample
 
 
 def log(*a):
@@ -48,6 +50,7 @@ def main():
     ap.add_argument("--samples", type=int, default=262144)
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--path", choices=("mfma", "valu"), default="mfma")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--plain-rounds", type=int, default=2048)
     ap.add_argument("--no-plain", action="store_true")
@@ -73,8 +76,10 @@ def main():
     ds = episdet.Dataset.from_synthetic(syn)
     sets = episdet.n_choose_k(n, order)
 
+    path = episdet.PATH_FUSED_MFMA if args.path == "mfma" else episdet.PATH_FUSED_VALU
+
     def step():
-        return ds.run(order, episdet.K2, path=episdet.PATH_FUSED_VALU, tile=args.tile, rebuild=True)
+        return ds.run(order, episdet.K2, path=path, tile=args.tile, rebuild=True)
 
     for i in range(args.warmup):
         r = step()
@@ -92,7 +97,15 @@ def main():
     sec = max(times)
     value = sets * N / sec
     kernel_s = sum(kms) / len(kms) / 1e3
-    achieved = OPS_PER_SET_SAMPLE[order] * sets * N / kernel_s
+    if args.path == "mfma":
+        achieved = FLOPS_PER_SET_SAMPLE[order] * sets * N / kernel_s
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_FP4_PEAK / 1e12,
+                "unit": "TFLOP/s (dense FP4 MFMA, 16 flop per set-sample)", "frac": achieved / MFMA_FP4_PEAK}
+    else:
+        achieved = OPS_PER_SET_SAMPLE[order] * sets * N / kernel_s
+        roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12,
+                "unit": "T lane-ops/s (32-bit VALU, 0.5 per set-sample)", "frac": achieved / VALU_PEAK_OPS}
+    roof.update({"traffic": None, "kernel_ms": kernel_s * 1e3})
     r = results[-1]
     planted_ok = tuple(r["idx"][:order]) == syn.planted[0]
 
@@ -141,15 +154,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 (bit-plane AND+popcount, exact int counts; FP32 scoring, FP64 re-score)",
+        "dtype": ("fp4 e2m1 0/1 operands, fp32 exact-integer accumulation" if args.path == "mfma" else
+                  "u32 bit-plane AND+popcount") + "; exact int counts, FP32 scoring, FP64 re-score",
         "data": "synthetic",
         "config": {"workload": f"3-way K2 exhaustive search, {n} SNPs x {N} samples ({Nc}/{Nt}), HWE, "
                                f"MAF~U[0.05,0.5], 1 planted triplet, seed {args.seed}",
-                   "snps": n, "cases": Nc, "controls": Nt, "sets": sets, "path": "fused_valu",
+                   "snps": n, "cases": Nc, "controls": Nt, "sets": sets, "path": "fused_" + args.path,
                    "tile": r["tile"], "parallelism": "1 GPU"},
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12,
-                     "unit": "T lane-ops/s (32-bit VALU)", "frac": achieved / VALU_PEAK_OPS, "traffic": None,
-                     "kernel_ms": kernel_s * 1e3},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "plain_path": plain,
         "best": {"idx": list(r["idx"][:order]), "k2": r["best_score"], "planted_recovered": planted_ok,

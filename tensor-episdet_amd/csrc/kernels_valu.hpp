@@ -78,7 +78,8 @@ struct CandOut {
 // two is appended.  Since the global minimum only decreases, the final
 // argmin of the exact scores is always among the appended sets when margin
 // is twice the FP32 error bound.
-template <int M>
+// NW = waves per workgroup; s_red holds at least NW + 1 floats.
+template <int M, int NW = 4>
 __device__ __forceinline__ void publish_candidates(const CandOut &co, float margin, const float (&bad)[M],
                                                    const unsigned long long (&packed)[M], float *s_red)
 {
@@ -90,16 +91,18 @@ __device__ __forceinline__ void publish_candidates(const CandOut &co, float marg
     if ((tid & 63) == 0) s_red[tid >> 6] = wmin;
     __syncthreads();
     if (tid == 0) {
-        float b = fminf(fminf(s_red[0], s_red[1]), fminf(s_red[2], s_red[3]));
+        float b = s_red[0];
+#pragma unroll
+        for (int w = 1; w < NW; w++) b = fminf(b, s_red[w]);
         float thr = -INFINITY;
         if (b < INFINITY) {
             unsigned old = atomicMin(co.best_key, f2key(b));
             thr = fminf(key2f(old), b) + margin;
         }
-        s_red[4] = thr;
+        s_red[NW] = thr;
     }
     __syncthreads();
-    const float thr = s_red[4];
+    const float thr = s_red[NW];
 #pragma unroll
     for (int i = 0; i < M; i++) {
         if (bad[i] <= thr) {

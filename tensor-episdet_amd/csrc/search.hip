@@ -19,7 +19,7 @@
 #include <vector>
 
 #include "episdet.h"
-#include "kernels_valu.hpp"
+#include "kernels_mfma.hpp"
 
 using namespace episdet;
 
@@ -229,6 +229,22 @@ int dispatch_tri(episdet_dataset *ds, int tile, TriArgs &a, int grid_override, h
 }
 
 template <int MODE>
+int dispatch_tri_mfma(episdet_dataset *ds, TriArgs &a, hipStream_t s)
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_triplets_mfma<MODE>, 512, 0) != hipSuccess ||
+        per < 1)
+        per = 1;
+    hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
+    HIPCHK(hipGetLastError());
+    return EPISDET_OK;
+}
+
+// The MFMA path accumulates counts in FP32: exact while every class has
+// fewer than 2^24 samples.
+bool mfma_ok(const episdet_dataset *ds) { return ds->N[0] < (1 << 24) && ds->N[1] < (1 << 24); }
+
+template <int MODE>
 int dispatch_pairs(episdet_dataset *ds, int tile, PairArgs &a, hipStream_t s)
 {
     const int R = PAIR_TILES[tile];
@@ -405,7 +421,9 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
 int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
 {
     const int order = o->order, score = o->score;
-    const int tile = o->tile;
+    const bool mfma = o->path == EPISDET_PATH_FUSED_MFMA;
+    if (mfma && (order != 3 || !mfma_ok(ds))) return EPISDET_ERR_INVALID;
+    const int tile = mfma ? 0 : o->tile;
     if (order == 3 && (tile < 0 || tile >= (int)(sizeof TRI_TILES / sizeof TRI_TILES[0]))) return EPISDET_ERR_INVALID;
     if (order == 2 && (tile < 0 || tile >= (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0]))) return EPISDET_ERR_INVALID;
     CHK(ensure_tables(ds, score));
@@ -446,7 +464,8 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, ds->stream));
-    int rc = order == 3 ? dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream)
+    int rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, ta, ds->stream)
+                                : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
                         : dispatch_pairs<MODE_SEARCH>(ds, tile, pa, ds->stream);
     HIPCHK(hipEventRecord(e1, ds->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -476,7 +495,7 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
         CHK(run_plain(ds, &po, res));
         res->kernel_ms += k;
         res->fallback = 1;
-        res->path = EPISDET_PATH_FUSED_VALU;
+        res->path = o->path;
         return EPISDET_OK;
     }
     const int count = (int)hc.cand_count;
@@ -510,7 +529,7 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     unpack(order, bp, res->idx);
     res->fp32_max_abs_err = maxabs;
     res->fp32_max_rel_err = maxrel;
-    res->path = EPISDET_PATH_FUSED_VALU;
+    res->path = mfma ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
     return EPISDET_OK;
 }
 
@@ -725,7 +744,7 @@ int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int 
     if (rc == EPISDET_OK && hipMalloc(&ds->ctl, sizeof(Ctl)) != hipSuccess) rc = EPISDET_ERR_ALLOC;
     for (int c = 0; c < 2 && rc == EPISDET_OK; c++) {
         const int Wv = (ds->N[c] + 31) / 32;
-        ds->W[c] = ((Wv + KW - 1) / KW) * KW;
+        ds->W[c] = ((Wv + MF_KW - 1) / MF_KW) * MF_KW;   // a multiple of every kernel's chunk
         const size_t in_bytes = sizeof(uint32_t) * (size_t)numSNPs * 2 * in_w[c];
         const size_t dev_words = (size_t)ds->nrows * 2 * ds->W[c];
         uint32_t *stage = nullptr;
@@ -794,7 +813,8 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
     int rc;
     switch (opt->path) {
     case EPISDET_PATH_AUTO:
-    case EPISDET_PATH_FUSED_VALU: rc = run_fused(ds, opt, res); break;
+    case EPISDET_PATH_FUSED_VALU:
+    case EPISDET_PATH_FUSED_MFMA: rc = run_fused(ds, opt, res); break;
     case EPISDET_PATH_PLAIN: rc = run_plain(ds, opt, res); break;
     default: rc = EPISDET_ERR_INVALID;
     }
@@ -888,7 +908,10 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
                             unsigned long long *checksum)
 {
     if (!ds || (order != 2 && order != 3)) return EPISDET_ERR_INVALID;
-    if (path != EPISDET_PATH_FUSED_VALU && path != EPISDET_PATH_AUTO) return EPISDET_ERR_INVALID;
+    if (path != EPISDET_PATH_FUSED_VALU && path != EPISDET_PATH_AUTO && path != EPISDET_PATH_FUSED_MFMA)
+        return EPISDET_ERR_INVALID;
+    const bool mfma = path == EPISDET_PATH_FUSED_MFMA;
+    if (mfma && (order != 3 || !mfma_ok(ds))) return EPISDET_ERR_INVALID;
     const int nc = order == 3 ? 27 : 9;
     const unsigned long long total = episdet_n_choose_k((unsigned)ds->n, (unsigned)order);
     uint32_t *d_dump = nullptr;
@@ -918,7 +941,11 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
         ta.ngroups = (int)groups.size();
         ta.nunits = gstart.back();
         ta.dump = d_dump;
-        rc = out ? dispatch_tri<MODE_DUMP>(ds, 0, ta, 0, ds->stream) : dispatch_tri<MODE_CHECKSUM>(ds, 0, ta, 0, ds->stream);
+        if (mfma)
+            rc = out ? dispatch_tri_mfma<MODE_DUMP>(ds, ta, ds->stream) : dispatch_tri_mfma<MODE_CHECKSUM>(ds, ta, ds->stream);
+        else
+            rc = out ? dispatch_tri<MODE_DUMP>(ds, 0, ta, 0, ds->stream)
+                     : dispatch_tri<MODE_CHECKSUM>(ds, 0, ta, 0, ds->stream);
     } else {
         PairArgs pa;
         fill_pair_common(ds, pa, 16 * PAIR_TILES[0]);

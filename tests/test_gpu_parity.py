@@ -80,13 +80,13 @@ CASES = [
 def test_fused_tables_bit_exact(n, Nc, Nt, wc, wt, seed):
     syn = episdet.generate(n, Nc, Nt, seed=seed, cases_words=wc, controls_words=wt)
     with episdet.Dataset.from_synthetic(syn) as ds:
-        for order in (2, 3):
-            say(f"dump order={order} n={n} Nc={Nc} Nt={Nt}")
-            dump = ds.dump_tables(order)
+        for order, path in ((2, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_MFMA)):
+            say(f"dump order={order} path={path} n={n} Nc={Nc} Nt={Nt}")
+            dump = ds.dump_tables(order, path=path)
             ref, cs = O.all_tables(syn.cases, syn.controls, Nc, Nt, order)
             assert (dump == ref).all()
-            say(f"checksum order={order}")
-            assert ds.checksum(order) == cs
+            say(f"checksum order={order} path={path}")
+            assert ds.checksum(order, path=path) == cs
 
 
 @pytest.mark.parametrize("n,Nc,Nt,wc,wt,seed", CASES)
@@ -94,9 +94,9 @@ def test_fused_best_matches_oracle(n, Nc, Nt, wc, wt, seed):
     syn = episdet.generate(n, Nc, Nt, seed=seed, cases_words=wc, controls_words=wt,
                            planted_sets=1 if n >= 60 else 0, planted_order=3)
     with episdet.Dataset.from_synthetic(syn) as ds:
-        for order in (2, 3):
+        for order, path in ((2, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_MFMA)):
             for score in (episdet.K2, episdet.MI):
-                r = check_best(ds, syn.cases, syn.controls, Nc, Nt, order, score)
+                r = check_best(ds, syn.cases, syn.controls, Nc, Nt, order, score, path=path)
                 if score == episdet.K2 and r["fallback"] == 0:
                     assert r["fp32_max_rel_err"] <= K2_FP32_REL_TOL
 
@@ -139,6 +139,16 @@ def test_minimum_sizes():
             check_best(ds, syn.cases, syn.controls, 100, 90, order, episdet.K2)
 
 
+def test_mfma_paths_agree_with_valu_and_oracle():
+    syn = episdet.generate(200, 5000, 3000, seed=17, planted_sets=1)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for score in (episdet.K2, episdet.MI):
+            a = check_best(ds, syn.cases, syn.controls, 5000, 3000, 3, score, path=episdet.PATH_FUSED_MFMA)
+            b = check_best(ds, syn.cases, syn.controls, 5000, 3000, 3, score, path=episdet.PATH_FUSED_VALU)
+            assert a["packed"] == b["packed"] and a["best_score"] == b["best_score"]
+        assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA) == ds.checksum(3, path=episdet.PATH_FUSED_VALU)
+
+
 def test_degenerate_data_all_homozygous_and_duplicates():
     # every SNP monomorphic (all genotype 0): every table has one full cell,
     # all scores tie exactly, the lowest packed index must win.
@@ -151,12 +161,12 @@ def test_degenerate_data_all_homozygous_and_duplicates():
         if rem:
             arr[:, 0, full] = (0xFFFFFFFF << (32 - rem)) & 0xFFFFFFFF
     with episdet.Dataset(cases, controls, Nc, Nt) as ds:
-        for order in (2, 3):
-            dump = ds.dump_tables(order)
+        for order, path in ((2, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_MFMA)):
+            dump = ds.dump_tables(order, path=path)
             ref, _ = O.all_tables(cases, controls, Nc, Nt, order)
             assert (dump == ref).all()
             for score in (episdet.K2, episdet.MI):
-                r = check_best(ds, cases, controls, Nc, Nt, order, score)
+                r = check_best(ds, cases, controls, Nc, Nt, order, score, path=path)
                 assert r["packed"] == packed_of(order, tuple(range(order)))
 
 
@@ -192,9 +202,12 @@ def test_full_sample_axis_planted_triplet():
     identical to the oracle, the planted triplet recovered, FP32 within tol."""
     syn = episdet.generate(192, 131072, 131072, seed=2, planted_sets=1, planted_order=3)
     with episdet.Dataset.from_synthetic(syn) as ds:
-        r = check_best(ds, syn.cases, syn.controls, 131072, 131072, 3, episdet.K2)
-        assert tuple(r["idx"]) == syn.planted[0]
-        assert r["fp32_max_rel_err"] <= K2_FP32_REL_TOL
+        for path in (episdet.PATH_FUSED_VALU, episdet.PATH_FUSED_MFMA):
+            r = check_best(ds, syn.cases, syn.controls, 131072, 131072, 3, episdet.K2, path=path)
+            assert tuple(r["idx"]) == syn.planted[0]
+            assert r["fp32_max_rel_err"] <= K2_FP32_REL_TOL
+        say("checksums at the full sample axis: MFMA vs VALU")
+        assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA) == ds.checksum(3, path=episdet.PATH_FUSED_VALU)
         say("checksum at full sample axis vs plain tables")
         sets = np.array(list(itertools.islice(itertools.combinations(range(192), 3), 0, 4000, 7)), dtype=np.int32)
         t = ds.tables(3, sets)
