@@ -126,14 +126,19 @@ def main():
         # calibrate a slice of the search (largest index in [z0, z0+zn)) to
         # about --cpu-seconds: the marginal cost of one more Z from two probes
         # (each probe also pays for its pair table, which cancels out)
+        # The oracle parallelises over the largest index, so each probe spans
+        # a multiple of the thread count.
         z0 = max(order, min(n // 4, 600))
         probe_t = []
-        for k in (4, 12):
+        ks = (threads, 3 * threads)
+        for k in ks:
             t0 = time.perf_counter()
-            oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + k))
+            oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads,
+                          last=(z0, min(n, z0 + k)))
             probe_t.append(time.perf_counter() - t0)
-        per_z = max((probe_t[1] - probe_t[0]) / 8, 1e-4)
-        zn = int(max(4, min(n - z0, args.cpu_seconds / per_z)))
+        per_z = max((probe_t[1] - probe_t[0]) / (ks[1] - ks[0]), probe_t[0] / ks[0] / 4, 1e-4)
+        zn = int(max(threads, min(n - z0, 4 * threads * 64, args.cpu_seconds / per_z)))
+        log(f"cpu: probes {probe_t[0]:.2f} s / {probe_t[1]:.2f} s, timing largest index in [{z0},{z0 + zn})")
         t0 = time.perf_counter()
         c = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + zn))
         dt = time.perf_counter() - t0
