@@ -20,7 +20,7 @@ plain      = the reference-shaped round loop (count kernel -> int32 tables in
 cpu        = the OpenMP AND+popcount oracle on a contiguous slice of the same
              search (sets whose largest index lies in a stated range)
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--n SNPS] [--samples N]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3|5] [--path mfma|valu]
 """
 import argparse
 import json
@@ -37,6 +37,21 @@ MFMA_FP4_PEAK = 10.1e15                  # dense FP4 MFMA flop/s, MI355X
 FLOPS_PER_SET_SAMPLE = {2: 8.0, 3: 16.0} # MFMA: 2^k multiply-adds per set-sample
 
 
+SCORE_NAME = {0: "K2", 1: "MI"}
+# BASELINE configs 1, 2, 3 and 5 (config 4, the batched jobs, is not a
+# single search).  Config 3's classes are 26214 / 39322 of 65536.
+PRESETS = {
+    1: dict(order=2, score=0, n=512, samples=2048, case_frac=0.5, planted=0, maf=(0.05, 0.5), maf_dist=0,
+            maf_text="U[0.05,0.5]"),
+    2: dict(order=3, score=0, n=4096, samples=262144, case_frac=0.5, planted=1, maf=(0.05, 0.5), maf_dist=0,
+            maf_text="U[0.05,0.5]"),
+    3: dict(order=2, score=1, n=16384, samples=65536, case_frac=26214 / 65536, planted=1, maf=(0.01, 0.5),
+            maf_dist=1, maf_text="Beta(0.5,2) clipped to [0.01,0.5]"),
+    5: dict(order=3, score=0, n=8192, samples=65536, case_frac=0.5, planted=2, maf=(0.05, 0.5), maf_dist=0,
+            maf_text="U[0.05,0.5]"),
+}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -46,9 +61,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
-    ap.add_argument("--n", type=int, default=4096)
-    ap.add_argument("--samples", type=int, default=262144)
-    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 5),
+                    help="workload preset (BASELINE configs); 2 is the headline")
+    ap.add_argument("--n", type=int, default=0, help="override the preset's SNP count")
+    ap.add_argument("--samples", type=int, default=0, help="override the preset's sample count")
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--path", choices=("mfma", "valu"), default="mfma")
     ap.add_argument("--seed", type=int, default=1)
@@ -64,12 +80,17 @@ def main():
 
     import episdet
 
-    n, N = args.n, args.samples
-    Nc = N // 2
+    pre = PRESETS[args.config]
+    n = args.n or pre["n"]
+    N = args.samples or pre["samples"]
+    Nc = int(round(N * pre["case_frac"]))
     Nt = N - Nc
-    order = args.order
+    order, score = pre["order"], pre["score"]
+    if order == 2:
+        args.path = "valu"
     t = time.time()
-    syn = episdet.generate(n, Nc, Nt, seed=args.seed, planted_sets=1, planted_order=order)
+    syn = episdet.generate(n, Nc, Nt, seed=args.seed, planted_sets=pre["planted"], planted_order=order,
+                           maf=pre["maf"], maf_dist=pre["maf_dist"])
     log(f"generated {n} SNPs x {N} samples in {time.time() - t:.1f} s; planted {syn.planted}")
     name, cu, clk = episdet.device_info()
     log(f"device {name}, {cu} CUs, {clk / 1e3:.0f} MHz")
@@ -79,7 +100,7 @@ def main():
     path = episdet.PATH_FUSED_MFMA if args.path == "mfma" else episdet.PATH_FUSED_VALU
 
     def step():
-        return ds.run(order, episdet.K2, path=path, tile=args.tile, rebuild=True)
+        return ds.run(order, score, path=path, tile=args.tile, rebuild=True)
 
     for i in range(args.warmup):
         r = step()
@@ -107,11 +128,11 @@ def main():
                 "unit": "T lane-ops/s (32-bit VALU, 0.5 per set-sample)", "frac": achieved / VALU_PEAK_OPS}
     roof.update({"traffic": None, "kernel_ms": kernel_s * 1e3})
     r = results[-1]
-    planted_ok = tuple(r["idx"][:order]) == syn.planted[0]
+    planted_ok = (tuple(r["idx"][:order]) in syn.planted) if syn.planted else None
 
     plain = None
     if not args.no_plain and order == 3:
-        rr = ds.run(order, episdet.K2, path=episdet.PATH_PLAIN, plain_rounds=args.plain_rounds)
+        rr = ds.run(order, score, path=episdet.PATH_PLAIN, plain_rounds=args.plain_rounds)
         total_rounds = rr["rounds"]
         est = rr["kernel_ms"] / 1e3 * total_rounds / rr["rounds_run"]
         plain = {"value": sets * N / est, "rounds_timed": rr["rounds_run"], "rounds_total": total_rounds,
@@ -133,23 +154,24 @@ def main():
         ks = (threads, 3 * threads)
         for k in ks:
             t0 = time.perf_counter()
-            oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads,
+            oracle.search(syn.cases, syn.controls, Nc, Nt, order, score, threads=threads,
                           last=(z0, min(n, z0 + k)))
             probe_t.append(time.perf_counter() - t0)
         per_z = max((probe_t[1] - probe_t[0]) / (ks[1] - ks[0]), probe_t[0] / ks[0] / 4, 1e-4)
         zn = int(max(threads, min(n - z0, 4 * threads * 64, args.cpu_seconds / per_z)))
         log(f"cpu: probes {probe_t[0]:.2f} s / {probe_t[1]:.2f} s, timing largest index in [{z0},{z0 + zn})")
         t0 = time.perf_counter()
-        c = oracle.search(syn.cases, syn.controls, Nc, Nt, order, oracle.K2, threads=threads, last=(z0, z0 + zn))
+        c = oracle.search(syn.cases, syn.controls, Nc, Nt, order, score, threads=threads, last=(z0, z0 + zn))
         dt = time.perf_counter() - t0
         cpu = {"value": c["sets"] * N / dt, "unit": "set-samples/s", "cores": threads, "kind": "port",
                "sample": f"oracle (OpenMP AND+popcount{', AVX-512 VPOPCNTDQ' if oracle.uses_avx512() else ''}, "
-                         f"FP64 K2) on the {c['sets']} triplets whose largest index is in [{z0},{z0 + zn}), "
-                         f"{dt:.1f} s"}
+                         f"FP64 {SCORE_NAME[score]}) on the {c['sets']} sets whose largest index is in "
+                         f"[{z0},{z0 + zn}), {dt:.1f} s"}
         log(f"cpu: {c['sets']} sets in {dt:.1f} s on {threads} threads")
 
     out = {
-        "metric": "3-way K2 sets x samples/s (fused FP32-scored search, end to end, data resident in HBM)",
+        "metric": f"{order}-way {SCORE_NAME[score]} sets x samples/s (fused FP32-scored search, end to end, "
+                  "data resident in HBM)",
         "value": value,
         "unit": "set-samples/s",
         "n_gpus": 1,
@@ -162,14 +184,16 @@ def main():
         "dtype": ("fp4 e2m1 0/1 operands, fp32 exact-integer accumulation" if args.path == "mfma" else
                   "u32 bit-plane AND+popcount") + "; exact int counts, FP32 scoring, FP64 re-score",
         "data": "synthetic",
-        "config": {"workload": f"3-way K2 exhaustive search, {n} SNPs x {N} samples ({Nc}/{Nt}), HWE, "
-                               f"MAF~U[0.05,0.5], 1 planted triplet, seed {args.seed}",
+        "config": {"workload": f"{order}-way {SCORE_NAME[score]} exhaustive search, {n} SNPs x {N} samples "
+                               f"({Nc}/{Nt}), HWE, MAF~{pre['maf_text']}, {pre['planted']} planted "
+                               f"{'triplet' if order == 3 else 'pair'}(s), seed {args.seed}",
+                   "preset": args.config,
                    "snps": n, "cases": Nc, "controls": Nt, "sets": sets, "path": "fused_" + args.path,
                    "tile": r["tile"], "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_path": plain,
-        "best": {"idx": list(r["idx"][:order]), "k2": r["best_score"], "planted_recovered": planted_ok,
+        "best": {"idx": list(r["idx"][:order]), "score": r["best_score"], "planted_recovered": planted_ok,
                  "candidates": r["candidates"], "fp32_max_rel_err": r["fp32_max_rel_err"]},
     }
     print(json.dumps(out), flush=True)

@@ -151,12 +151,13 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(TriArgs p)
     const int arow0 = (wy * 16) * 4 + i, arow1 = (wy * 16 + 8) * 4 + i;
     const int brow0 = MF_TY * 4 + (wz * 32) * 2 + i, brow1 = MF_TY * 4 + (wz * 32 + 16) * 2 + i;
 
+    int g = 0;
     for (;;) {
         if (tid == 0) s_u = atomicAdd(p.queue, 1ull);
         __syncthreads();
         const unsigned long long u = s_u;
         if (u >= p.nunits) break;
-        const int g = find_group(p.gstart, p.ngroups, u);
+        g = next_group(p.gstart, p.ngroups, u, g);
         const int X = (int)(u - p.gstart[g]);
         const int2 gt = p.groups[g];
         const int y0 = gt.x * MF_TY, z0 = gt.y * MF_TZ;
@@ -177,18 +178,32 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(TriArgs p)
             __syncthreads();
             if (c < nch_mine) {
                 if (c + 1 < nch_mine) mf_load(base, W, X, y0, z0, (c + 1) * MF_KW, t, st);
+                // operands of superstep s+1 are read while superstep s runs
+                uint4 a[2], b[2];
+                a[0] = *(const uint4 *)&rows[arow0][h * 4];
+                a[1] = *(const uint4 *)&rows[arow1][h * 4];
+                b[0] = *(const uint4 *)&rows[brow0][h * 4];
+                b[1] = *(const uint4 *)&rows[brow1][h * 4];
 #pragma unroll
                 for (int s = 0; s < MF_KW / 8; s++) {
-                    const int w = s * 8 + h * 4;
-                    uint4 a[2], b[2];
-                    a[0] = *(const uint4 *)&rows[arow0][w];
-                    a[1] = *(const uint4 *)&rows[arow1][w];
-                    b[0] = *(const uint4 *)&rows[brow0][w];
-                    b[1] = *(const uint4 *)&rows[brow1][w];
+                    uint4 an[2], bn[2];
+                    if (s + 1 < MF_KW / 8) {
+                        const int w = (s + 1) * 8 + h * 4;
+                        an[0] = *(const uint4 *)&rows[arow0][w];
+                        an[1] = *(const uint4 *)&rows[arow1][w];
+                        bn[0] = *(const uint4 *)&rows[brow0][w];
+                        bn[1] = *(const uint4 *)&rows[brow1][w];
+                    }
                     mf_step<0>(a, b, acc);
                     mf_step<1>(a, b, acc);
                     mf_step<2>(a, b, acc);
                     mf_step<3>(a, b, acc);
+                    if (s + 1 < MF_KW / 8) {
+                        a[0] = an[0];
+                        a[1] = an[1];
+                        b[0] = bn[0];
+                        b[1] = bn[1];
+                    }
                 }
             }
         }
@@ -245,6 +260,233 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(TriArgs p)
             }
         }
         if constexpr (MODE == MODE_SEARCH) publish_candidates<4, 8>(p.co, p.sc.margin, bad, packed, s_red);
+        if constexpr (MODE == MODE_CHECKSUM) {
+            if (hsum) atomicAdd(p.checksum, hsum);
+        }
+    }
+}
+
+
+// ============================================================ version 2 ==
+//
+// Same contraction and FP4 encoding, restructured so the chunk pipeline costs
+// no VALU work and one barrier:
+//  * the X, Y and Z rows of a 1024-sample chunk go straight from global
+//    memory to LDS (global_load_lds_dwordx4), into one of two buffers while
+//    the other is contracted;
+//  * LDS rows are 128 bytes without padding; the eight 16-byte slots of row r
+//    are XOR-swizzled by r & 7, so the lanes of a wave, which read up to eight
+//    consecutive rows at the same logical slot, hit eight different banks;
+//  * P_ij = X_i & Y_j is formed in the operand itself (one v_bitop3 per
+//    dword together with the class mask);
+//  * a work unit is X x 64 Y x 64 Z; wave (class, wy) owns 16 Y x 64 Z =
+//    2 x 4 MFMA tiles (128 accumulators), so each A operand feeds four MFMAs
+//    and each B operand two.
+// After the last chunk the accumulators go to LDS in two halves (Z 0-31,
+// then Z 32-63) and all 512 threads score four triplets per half.
+
+constexpr int M2_TY = 64, M2_TZ = 64;
+constexpr int M2_KW = 32;                                   // words per chunk
+constexpr int M2_ROWS = 2 + 2 * M2_TY + 2 * M2_TZ;          // X, Y, Z rows of a class
+constexpr int M2_ROWS_PAD = (M2_ROWS + 7) / 8 * 8;
+constexpr int M2_LOADS = M2_ROWS_PAD / 8;                   // 8 rows per load instruction
+constexpr int M2_BUF_WORDS = M2_ROWS_PAD * M2_KW;
+constexpr int M2_STAGE_WORDS = 4 * M2_BUF_WORDS;            // 2 classes x 2 buffers
+constexpr int M2_DUMP_WORDS = 2 * 4 * 4 * 16 * 64;          // class x wave x tile x reg x lane
+constexpr int M2_SMEM_WORDS = M2_STAGE_WORDS > M2_DUMP_WORDS ? M2_STAGE_WORDS : M2_DUMP_WORDS;
+constexpr int M2_YROW = 2, M2_ZROW = 2 + 2 * M2_TY;
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Global row of LDS row r (X planes, then Y rows 2y+j, then Z rows 2z+k).
+__device__ __forceinline__ const uint32_t *m2_rowptr(const uint32_t *base, int W, int X, int y0, int z0, int r)
+{
+    if (r < M2_YROW) return base + ((size_t)X * 2 + r) * W;
+    if (r < M2_ZROW) {
+        const int rr = r - M2_YROW;
+        return base + ((size_t)(y0 + (rr >> 1)) * 2 + (rr & 1)) * W;
+    }
+    if (r < M2_ROWS) {
+        const int rr = r - M2_ZROW;
+        return base + ((size_t)(z0 + (rr >> 1)) * 2 + (rr & 1)) * W;
+    }
+    return base + (size_t)X * 2 * W;     // padding rows are never read
+}
+
+// This wave's share of the direct-to-LDS loads of chunk w0 into buf.  One
+// instruction fills eight rows: lane L writes slot L of a 1 KB block, i.e.
+// row 8*ii + L/8, physical slot L%8, which holds logical slot (L%8) ^ (L/8).
+__device__ __forceinline__ void m2_issue(const uint32_t *base, int W, int X, int y0, int z0, int w0, uint32_t *buf,
+                                         int wih, int lane)
+{
+    const int rl = lane >> 3, q = (lane & 7) ^ rl;
+#pragma unroll
+    for (int ii = wih; ii < M2_LOADS; ii += 4) {
+        const uint32_t *src = m2_rowptr(base, W, X, y0, z0, ii * 8 + rl) + w0 + q * 4;
+        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)(buf + ii * 8 * M2_KW), 16, 0, 0);
+    }
+}
+
+// Word offset of logical 16-byte slot q of LDS row r.
+__device__ __forceinline__ int m2_at(int r, int q) { return r * M2_KW + ((q ^ (r & 7)) << 2); }
+
+template <int C>
+__device__ __forceinline__ v8i_t m2_aop(const uint4 &x, const uint4 &y)
+{
+    v8i_t o = {0, 0, 0, 0, 0, 0, 0, 0};
+    if constexpr (C < 3) {
+        constexpr uint32_t m = 0x11111111u << C;
+        o[0] = (int)(x.x & y.x & m);
+        o[1] = (int)(x.y & y.y & m);
+        o[2] = (int)(x.z & y.z & m);
+        o[3] = (int)(x.w & y.w & m);
+    } else {
+        o[0] = (int)(((x.x & y.x) >> 1) & 0x44444444u);
+        o[1] = (int)(((x.y & y.y) >> 1) & 0x44444444u);
+        o[2] = (int)(((x.z & y.z) >> 1) & 0x44444444u);
+        o[3] = (int)(((x.w & y.w) >> 1) & 0x44444444u);
+    }
+    return o;
+}
+
+template <int C>
+__device__ __forceinline__ void m2_step(const uint4 &xw, const uint4 (&yw)[2], const uint4 (&zw)[4],
+                                        v16f_t (&acc)[2][4])
+{
+    constexpr int s = mf_scale<C>();
+    const v8i_t A0 = m2_aop<C>(xw, yw[0]), A1 = m2_aop<C>(xw, yw[1]);
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+        const v8i_t B = mf_operand<C>(zw[cb]);
+        acc[0][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A0, B, acc[0][cb], 4, 4, 0, s, 0, s);
+        acc[1][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A1, B, acc[1][cb], 4, 4, 0, s, 0, s);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(512) void k_triplets_mfma2(TriArgs p)
+{
+    __shared__ __attribute__((aligned(16))) uint32_t smem[M2_SMEM_WORDS];
+    __shared__ unsigned long long s_u;
+    __shared__ float s_red[16];
+    const int tid = threadIdx.x;
+    const int cls = tid >> 8, t = tid & 255;
+    const int wy = t >> 6, lane = t & 63, h = lane >> 5, i = lane & 31;
+    uint32_t *buf0 = smem + (cls * 2 + 0) * M2_BUF_WORDS;
+    uint32_t *buf1 = smem + (cls * 2 + 1) * M2_BUF_WORDS;
+    const uint32_t *base = cls ? p.p1 : p.p0;
+    const int W = cls ? p.W1 : p.W0;
+    const int nch_mine = W / M2_KW;
+    const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / M2_KW;
+    // LDS rows read by this lane: X plane, Y rows of its two row blocks, Z
+    // rows of its four column blocks
+    const int xr = (i >> 1) & 1;
+    const int yr0 = M2_YROW + 2 * (wy * 16 + (i >> 2)) + (i & 1);
+    const int yr1 = M2_YROW + 2 * (wy * 16 + 8 + (i >> 2)) + (i & 1);
+    int zr[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) zr[cb] = M2_ZROW + 2 * (cb * 16 + (i >> 1)) + (i & 1);
+
+    int g = 0;
+    for (;;) {
+        if (tid == 0) s_u = atomicAdd(p.queue, 1ull);
+        __syncthreads();
+        const unsigned long long u = s_u;
+        if (u >= p.nunits) break;
+        g = next_group(p.gstart, p.ngroups, u, g);
+        const int X = (int)(u - p.gstart[g]);
+        const int2 gt = p.groups[g];
+        const int y0 = gt.x * M2_TY, z0 = gt.y * M2_TZ;
+
+        v16f_t acc[2][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
+
+        if (nch_mine > 0) m2_issue(base, W, X, y0, z0, 0, buf0, wy, lane);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int c = 0; c < nch; c++) {
+            uint32_t *cur = (c & 1) ? buf1 : buf0;
+            if (c + 1 < nch_mine) m2_issue(base, W, X, y0, z0, (c + 1) * M2_KW, (c & 1) ? buf0 : buf1, wy, lane);
+            if (c < nch_mine) {
+#pragma unroll
+                for (int s = 0; s < M2_KW / 8; s++) {
+                    const int q = 2 * s + h;
+                    const uint4 xw = *(const uint4 *)&cur[m2_at(xr, q)];
+                    uint4 yw[2], zw[4];
+                    yw[0] = *(const uint4 *)&cur[m2_at(yr0, q)];
+                    yw[1] = *(const uint4 *)&cur[m2_at(yr1, q)];
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++) zw[cb] = *(const uint4 *)&cur[m2_at(zr[cb], q)];
+                    m2_step<0>(xw, yw, zw, acc);
+                    m2_step<1>(xw, yw, zw, acc);
+                    m2_step<2>(xw, yw, zw, acc);
+                    m2_step<3>(xw, yw, zw, acc);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+
+        const Singles sx = p.singles[X];
+        float bad[8];
+        unsigned long long packed[8];
+        unsigned long long hsum = 0;
+#pragma unroll
+        for (int hf = 0; hf < 2; hf++) {
+            // tiles with column blocks 2hf, 2hf+1 -> LDS [cls][wy][t4][reg][lane]
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int cr = 0; cr < 2; cr++)
+#pragma unroll
+                    for (int r = 0; r < 16; r++)
+                        smem[((((cls * 4 + wy) * 4 + rb * 2 + cr) * 16) + r) * 64 + lane] =
+                            (uint32_t)acc[rb][2 * hf + cr][r];
+            __syncthreads();
+#pragma unroll
+            for (int q4 = 0; q4 < 4; q4++) {
+                const int m = hf * 4 + q4;
+                const int T = q4 * 512 + tid;                   // 64 Y x 32 Z of this half
+                const int yl = T >> 5, zl = hf * 32 + (T & 31);
+                const int Y = y0 + yl, Z = z0 + zl;
+                bad[m] = INFINITY;
+                packed[m] = (unsigned long long)X | ((unsigned long long)Y << 21) | ((unsigned long long)Z << 42);
+                if (!(X < Y && Y < Z && Z < p.n)) continue;
+                const int twy = yl >> 4, trb = (yl >> 3) & 1, y8 = yl & 7, tq = y8 >> 1, th = y8 & 1;
+                const int tcr = (T & 31) >> 4, z16 = T & 15;
+                const int t4 = trb * 2 + tcr;
+                uint32_t n8c[8], n8t[8];
+#pragma unroll
+                for (int ij = 0; ij < 4; ij++)
+#pragma unroll
+                    for (int kk = 0; kk < 2; kk++) {
+                        const int off = ((t4 * 16) + 4 * tq + ij) * 64 + th * 32 + 2 * z16 + kk;
+                        n8c[ij * 2 + kk] = smem[(0 * 4 + twy) * 4 * 16 * 64 + off];
+                        n8t[ij * 2 + kk] = smem[(1 * 4 + twy) * 4 * 16 * 64 + off];
+                    }
+                const Singles sy = p.singles[Y];
+                uint32_t cas[27], ctl[27];
+                triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
+                if constexpr (MODE == MODE_SEARCH) {
+                    bad[m] = badness_f32(p.sc, cas, ctl);
+                } else if constexpr (MODE == MODE_DUMP) {
+                    uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
+                    for (int c = 0; c < 27; c++) {
+                        o[c] = cas[c];
+                        o[27 + c] = ctl[c];
+                    }
+                } else if constexpr (MODE == MODE_CHECKSUM) {
+                    hsum += table_hash(rank3(X, Y, Z), cas, ctl, 27);
+                }
+            }
+            __syncthreads();
+        }
+        if constexpr (MODE == MODE_SEARCH) publish_candidates<8, 8>(p.co, p.sc.margin, bad, packed, s_red);
         if constexpr (MODE == MODE_CHECKSUM) {
             if (hsum) atomicAdd(p.checksum, hsum);
         }

@@ -286,6 +286,30 @@ struct TriArgs {
     uint32_t *counts;                    // [2][256][256][8]
 };
 
+// Group of unit u when the caller's previous unit was in group g0 <= answer
+// (a workgroup's units only increase): a short forward scan, then a binary
+// search over what is left if the unit jumped far ahead.
+__device__ __forceinline__ int find_group(const unsigned long long *gs, int ng, unsigned long long u);
+__device__ __forceinline__ int next_group(const unsigned long long *gs, int ng, unsigned long long u, int g0)
+{
+    int g = g0;
+#pragma unroll 1
+    for (int step = 0; step < 4; step++) {
+        if (g + 1 < ng && gs[g + 1] <= u) g++;
+        else return g;
+    }
+    if (g + 1 < ng && gs[g + 1] <= u) {
+        int lo = g, hi = ng;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (gs[mid] <= u) lo = mid;
+            else hi = mid;
+        }
+        g = lo;
+    }
+    return g;
+}
+
 __device__ __forceinline__ int find_group(const unsigned long long *gs, int ng, unsigned long long u)
 {
     int lo = 0, hi = ng;
@@ -397,6 +421,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
     __shared__ unsigned long long s_u;
     __shared__ float s_red[8];
     const int tid = threadIdx.x, ty = tid >> 4, tz = tid & 15;
+    int g = 0;
     for (int iter = 0;; iter++) {
         int X, y0, z0;
         if constexpr (MODE == MODE_COUNTS) {
@@ -410,7 +435,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
             __syncthreads();
             const unsigned long long u = s_u;
             if (u >= p.nunits) break;
-            const int g = find_group(p.gstart, p.ngroups, u);
+            g = next_group(p.gstart, p.ngroups, u, g);
             X = (int)(u - p.gstart[g]);
             const int2 gt = p.groups[g];
             y0 = gt.x * TY;

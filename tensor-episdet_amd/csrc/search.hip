@@ -228,14 +228,20 @@ int dispatch_tri(episdet_dataset *ds, int tile, TriArgs &a, int grid_override, h
     return EPISDET_ERR_INVALID;
 }
 
+// MFMA tiles: 0 = version 2 (64 x 64 units, direct-to-LDS double buffering),
+// 1 = version 1 (32 x 64 units, register-staged chunks).
+constexpr int MFMA_TILES = 2;
+inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : M2_TY; }
+inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : M2_TZ; }
+
 template <int MODE>
-int dispatch_tri_mfma(episdet_dataset *ds, TriArgs &a, hipStream_t s)
+int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
 {
+    const void *k = tile == 1 ? (const void *)k_triplets_mfma<MODE> : (const void *)k_triplets_mfma2<MODE>;
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_triplets_mfma<MODE>, 512, 0) != hipSuccess ||
-        per < 1)
-        per = 1;
-    hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 512, 0) != hipSuccess || per < 1) per = 1;
+    if (tile == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
+    else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
     HIPCHK(hipGetLastError());
     return EPISDET_OK;
 }
@@ -423,8 +429,10 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     const int order = o->order, score = o->score;
     const bool mfma = o->path == EPISDET_PATH_FUSED_MFMA;
     if (mfma && (order != 3 || !mfma_ok(ds))) return EPISDET_ERR_INVALID;
-    const int tile = mfma ? 0 : o->tile;
-    if (order == 3 && (tile < 0 || tile >= (int)(sizeof TRI_TILES / sizeof TRI_TILES[0]))) return EPISDET_ERR_INVALID;
+    const int tile = o->tile;
+    if (mfma && (tile < 0 || tile >= MFMA_TILES)) return EPISDET_ERR_INVALID;
+    if (!mfma && order == 3 && (tile < 0 || tile >= (int)(sizeof TRI_TILES / sizeof TRI_TILES[0])))
+        return EPISDET_ERR_INVALID;
     if (order == 2 && (tile < 0 || tile >= (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0]))) return EPISDET_ERR_INVALID;
     CHK(ensure_tables(ds, score));
     if (order == 3) CHK(ensure_pairs(ds));
@@ -438,8 +446,12 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     if (order == 3) {
         std::vector<int2> groups;
         std::vector<unsigned long long> gstart;
-        const Tri3Tile t = TRI_TILES[tile];
-        build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
+        if (mfma) {
+            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), groups, gstart);
+        } else {
+            const Tri3Tile t = TRI_TILES[tile];
+            build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
+        }
         if (groups.empty()) return EPISDET_ERR_INVALID;
         if (hipMalloc(&d_gstart, sizeof(unsigned long long) * gstart.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
         if (hipMalloc(&d_groups, sizeof(int2) * groups.size()) != hipSuccess) {
@@ -464,7 +476,7 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipEventRecord(e0, ds->stream));
-    int rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, ta, ds->stream)
+    int rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream)
                                 : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
                         : dispatch_pairs<MODE_SEARCH>(ds, tile, pa, ds->stream);
     HIPCHK(hipEventRecord(e1, ds->stream));
@@ -927,8 +939,12 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
     if (order == 3) {
         std::vector<int2> groups;
         std::vector<unsigned long long> gstart;
-        const Tri3Tile t = TRI_TILES[0];
-        build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
+        if (mfma) {
+            build_groups(ds->n, mfma_ty(0), mfma_tz(0), groups, gstart);
+        } else {
+            const Tri3Tile t = TRI_TILES[0];
+            build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
+        }
         if (hipMalloc(&d_gstart, sizeof(unsigned long long) * gstart.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
         if (hipMalloc(&d_groups, sizeof(int2) * std::max<size_t>(groups.size(), 1)) != hipSuccess) return EPISDET_ERR_ALLOC;
         HIPCHK(hipMemcpy(d_gstart, gstart.data(), sizeof(unsigned long long) * gstart.size(), hipMemcpyHostToDevice));
@@ -942,7 +958,8 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
         ta.nunits = gstart.back();
         ta.dump = d_dump;
         if (mfma)
-            rc = out ? dispatch_tri_mfma<MODE_DUMP>(ds, ta, ds->stream) : dispatch_tri_mfma<MODE_CHECKSUM>(ds, ta, ds->stream);
+            rc = out ? dispatch_tri_mfma<MODE_DUMP>(ds, 0, ta, ds->stream)
+                     : dispatch_tri_mfma<MODE_CHECKSUM>(ds, 0, ta, ds->stream);
         else
             rc = out ? dispatch_tri<MODE_DUMP>(ds, 0, ta, 0, ds->stream)
                      : dispatch_tri<MODE_CHECKSUM>(ds, 0, ta, 0, ds->stream);

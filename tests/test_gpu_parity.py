@@ -108,6 +108,13 @@ def test_triplet_tiles_agree(tile):
         check_best(ds, syn.cases, syn.controls, 1500, 1700, 3, episdet.K2, tile=tile)
 
 
+@pytest.mark.parametrize("tile", [0, 1])
+def test_mfma_versions_agree(tile):
+    syn = episdet.generate(170, 2500, 2100, seed=19, planted_sets=1)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        check_best(ds, syn.cases, syn.controls, 2500, 2100, 3, episdet.K2, path=episdet.PATH_FUSED_MFMA, tile=tile)
+
+
 def test_plain_path_matches_oracle():
     syn = episdet.generate(300, 1024, 1024, seed=12, planted_sets=1)
     with episdet.Dataset.from_synthetic(syn) as ds:
