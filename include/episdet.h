@@ -129,6 +129,33 @@ typedef struct {
 
 int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result *res);
 
+/* ---- batched searches (config "many independent jobs", no reference
+ * counterpart: the reference runs one search per process) ---- */
+
+typedef struct {
+    int order;                        /* 2 or 3                                  */
+    int score;                        /* episdet_score                           */
+    const unsigned *cases;            /* host planes as for episdet_dataset_create */
+    const unsigned *controls;
+    int numSNPs, numCases, numControls;
+    int casesWords, controlsWords;
+} episdet_job;
+
+/* Runs many independent searches in one hipGraph launch: every 3-way job's
+ * work units go through one persistent FP4-MFMA kernel launch, every 2-way
+ * job's through one persistent VALU kernel launch, followed by one FP64
+ * re-scoring kernel over all jobs' candidates.  Jobs the batch kernels cannot
+ * take (PLAIN path, 3-way with >= 2^24 samples per class, candidate
+ * overflow) run on their own afterwards.  opts[j] as for episdet_run (order
+ * and score per job); batch_ms, if not NULL, receives the device time of the
+ * graph.  Results are filled as episdet_run fills them. */
+int episdet_run_batch(episdet_dataset *const *ds, const episdet_options *opts, int njobs, episdet_result *results,
+                      double *batch_ms);
+
+/* Same, from host buffers: uploads every job, runs the batch, frees. */
+int episdet_search_batch(const episdet_job *jobs, int njobs, const episdet_options *opt, episdet_result *results,
+                         double *batch_ms);
+
 /* Tables of explicit sets, counted on the device by the same counting and
  * derivation code as the fused kernels.  sets: m x order indices, each row
  * strictly increasing.  out: m x 2 x 3^order u32 (cases cells, then controls

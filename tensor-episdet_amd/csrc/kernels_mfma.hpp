@@ -363,8 +363,19 @@ __device__ __forceinline__ void m2_step(const uint4 &xw, const uint4 (&yw)[2], c
     }
 }
 
+// A batch of independent 3-way searches sharing one work queue: units
+// [job_start[j], job_start[j+1]) belong to job j.  A single search is a batch
+// of one.
+struct BatchTri {
+    const TriArgs *jobs;                    // device array, one per job
+    const unsigned long long *job_start;    // [njobs + 1]
+    int njobs;
+    unsigned long long total;
+    unsigned long long *queue;
+};
+
 template <int MODE>
-__global__ __launch_bounds__(512) void k_triplets_mfma2(TriArgs p)
+__global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
 {
     __shared__ __attribute__((aligned(16))) uint32_t smem[M2_SMEM_WORDS];
     __shared__ unsigned long long s_u;
@@ -374,10 +385,6 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(TriArgs p)
     const int wy = t >> 6, lane = t & 63, h = lane >> 5, i = lane & 31;
     uint32_t *buf0 = smem + (cls * 2 + 0) * M2_BUF_WORDS;
     uint32_t *buf1 = smem + (cls * 2 + 1) * M2_BUF_WORDS;
-    const uint32_t *base = cls ? p.p1 : p.p0;
-    const int W = cls ? p.W1 : p.W0;
-    const int nch_mine = W / M2_KW;
-    const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / M2_KW;
     // LDS rows read by this lane: X plane, Y rows of its two row blocks, Z
     // rows of its four column blocks
     const int xr = (i >> 1) & 1;
@@ -387,12 +394,25 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(TriArgs p)
 #pragma unroll
     for (int cb = 0; cb < 4; cb++) zr[cb] = M2_ZROW + 2 * (cb * 16 + (i >> 1)) + (i & 1);
 
-    int g = 0;
+    int g = 0, j = 0;
     for (;;) {
-        if (tid == 0) s_u = atomicAdd(p.queue, 1ull);
+        if (tid == 0) s_u = atomicAdd(bt.queue, 1ull);
         __syncthreads();
-        const unsigned long long u = s_u;
-        if (u >= p.nunits) break;
+        const unsigned long long uu = s_u;
+        if (uu >= bt.total) break;
+        // job of this unit; a workgroup's units only increase
+        int jn = j;
+        while (jn + 1 < bt.njobs && bt.job_start[jn + 1] <= uu) jn++;
+        if (jn != j) {
+            j = jn;
+            g = 0;
+        }
+        const TriArgs &p = bt.jobs[j];
+        const unsigned long long u = uu - bt.job_start[j];
+        const uint32_t *base = cls ? p.p1 : p.p0;
+        const int W = cls ? p.W1 : p.W0;
+        const int nch_mine = W / M2_KW;
+        const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / M2_KW;
         g = next_group(p.gstart, p.ngroups, u, g);
         const int X = (int)(u - p.gstart[g]);
         const int2 gt = p.groups[g];

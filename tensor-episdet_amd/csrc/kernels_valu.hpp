@@ -199,8 +199,18 @@ __device__ __forceinline__ void unrank_tri(unsigned long long u, int &ta, int &t
     ta = (int)(u - (unsigned long long)b * (unsigned long long)(b + 1) / 2);
 }
 
+// A batch of independent 2-way searches sharing one work queue (see
+// BatchTri); a single search or pair-table build is a batch of one.
+struct BatchPair {
+    const PairArgs *jobs;
+    const unsigned long long *job_start;
+    int njobs;
+    unsigned long long total;
+    unsigned long long *queue;
+};
+
 template <int R, int MODE>
-__global__ __launch_bounds__(256) void k_pairs_valu(PairArgs p)
+__global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
 {
     constexpr int T = 16 * R;
     __shared__ uint32_t sA[2][T][LDSW];
@@ -208,11 +218,15 @@ __global__ __launch_bounds__(256) void k_pairs_valu(PairArgs p)
     __shared__ unsigned long long s_u;
     __shared__ float s_red[8];
     const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
+    int j = 0;
     for (;;) {
-        if (tid == 0) s_u = atomicAdd(p.queue, 1ull);
+        if (tid == 0) s_u = atomicAdd(bp.queue, 1ull);
         __syncthreads();
-        const unsigned long long u = s_u;
-        if (u >= p.nunits) break;
+        const unsigned long long uu = s_u;
+        if (uu >= bp.total) break;
+        while (j + 1 < bp.njobs && bp.job_start[j + 1] <= uu) j++;
+        const PairArgs &p = bp.jobs[j];
+        const unsigned long long u = uu - bp.job_start[j];
         int tA, tB;
         unrank_tri(u, tA, tB);
         const int a0 = tA * T, b0 = tB * T;
@@ -584,10 +598,9 @@ struct RescoreArgs {
 
 // One wave per set: counts the 2^k AND-cells of both classes straight from
 // the planes, derives every cell, scores in FP64.
-__global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)
+__device__ __forceinline__ void rescore_one(const RescoreArgs &p, int i)
 {
-    const int i = blockIdx.x, lane = threadIdx.x;
-    if (i >= p.count) return;
+    const int lane = threadIdx.x;
     const unsigned long long pk = p.cand[i];
     int s[3];
     if (p.order == 3) {
@@ -655,6 +668,48 @@ __global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)
     }
     if (p.out_s64)
         p.out_s64[i] = p.score == SCORE_K2 ? k2_f64(cas, ctl, nc, p.t64) : mi_f64(cas, ctl, nc, p.N0, p.N1, p.t64);
+}
+
+__global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)
+{
+    if ((int)blockIdx.x < p.count) rescore_one(p, blockIdx.x);
+}
+
+// FP64 re-scoring of the candidates of many searches; the candidate counts
+// are read on the device, so the whole batch needs no host round trip.
+struct RescoreJob {
+    RescoreArgs r;                 // r.count unused
+    const unsigned *count;         // candidates appended by the search
+    unsigned cap;                  // buffer capacity (larger counts: overflow)
+};
+
+__global__ __launch_bounds__(64) void k_rescore_batch(const RescoreJob *jobs, int njobs)
+{
+    for (unsigned long long gi = blockIdx.x;; gi += gridDim.x) {
+        unsigned long long start = 0;
+        int j = 0;
+        for (; j < njobs; j++) {
+            const unsigned c = *jobs[j].count;
+            const unsigned long long cnt = c > jobs[j].cap ? 0ull : (unsigned long long)c;
+            if (gi < start + cnt) break;
+            start += cnt;
+        }
+        if (j == njobs) return;
+        rescore_one(jobs[j].r, (int)(gi - start));
+    }
+}
+
+// Resets the control words of several searches at once (graph-capturable).
+__global__ void k_init_many(unsigned long long *const *queue, unsigned *const *best_key, unsigned *const *cand_count,
+                            int n, unsigned long long *batch_queues, int nq)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        *queue[i] = 0ull;
+        *best_key[i] = 0xFF800000u;
+        *cand_count[i] = 0u;
+    }
+    if (i < nq) batch_queues[i] = 0ull;
 }
 
 // Small fill kernels (no hipMemsetAsync, so the sequence stays capturable).

@@ -82,6 +82,8 @@ struct episdet_dataset {
     double *cand_s64 = nullptr;
     unsigned cand_cap = 0;
     int cu = 0;
+    void *jobargs = nullptr;                  // device copy of one kernel argument block
+    unsigned long long *jobstart = nullptr;   // {0, units} for a batch of one
 };
 
 namespace {
@@ -234,14 +236,37 @@ constexpr int MFMA_TILES = 2;
 inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : M2_TY; }
 inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : M2_TZ; }
 
+static_assert(sizeof(TriArgs) <= 1024 && sizeof(PairArgs) <= 1024, "argument slot too small");
+
+// Copies one argument block to the dataset's device slot, for a batch of one.
+template <typename A>
+int upload_single(episdet_dataset *ds, const A &a, unsigned long long units)
+{
+    const unsigned long long js[2] = {0ull, units};
+    HIPCHK(hipMemcpyAsync(ds->jobargs, &a, sizeof(A), hipMemcpyHostToDevice, ds->stream));
+    HIPCHK(hipMemcpyAsync(ds->jobstart, js, sizeof js, hipMemcpyHostToDevice, ds->stream));
+    HIPCHK(hipStreamSynchronize(ds->stream));
+    return EPISDET_OK;
+}
+
 template <int MODE>
 int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
 {
     const void *k = tile == 1 ? (const void *)k_triplets_mfma<MODE> : (const void *)k_triplets_mfma2<MODE>;
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 512, 0) != hipSuccess || per < 1) per = 1;
-    if (tile == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
-    else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
+    if (tile == 1) {
+        hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
+    } else {
+        CHK(upload_single(ds, a, a.nunits));
+        BatchTri bt;
+        bt.jobs = (const TriArgs *)ds->jobargs;
+        bt.job_start = ds->jobstart;
+        bt.njobs = 1;
+        bt.total = a.nunits;
+        bt.queue = a.queue;
+        hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(per * ds->cu), dim3(512), 0, s, bt);
+    }
     HIPCHK(hipGetLastError());
     return EPISDET_OK;
 }
@@ -251,18 +276,30 @@ int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
 bool mfma_ok(const episdet_dataset *ds) { return ds->N[0] < (1 << 24) && ds->N[1] < (1 << 24); }
 
 template <int MODE>
-int dispatch_pairs(episdet_dataset *ds, int tile, PairArgs &a, hipStream_t s)
+int launch_pairs_batch(int R, int cu, const BatchPair &bp, hipStream_t s)
 {
-    const int R = PAIR_TILES[tile];
     if (R == 4) {
-        int grid = launch_blocks((const void *)k_pairs_valu<4, MODE>, ds->cu);
-        hipLaunchKernelGGL((k_pairs_valu<4, MODE>), dim3(grid), dim3(256), 0, s, a);
+        int grid = launch_blocks((const void *)k_pairs_valu<4, MODE>, cu);
+        hipLaunchKernelGGL((k_pairs_valu<4, MODE>), dim3(grid), dim3(256), 0, s, bp);
     } else {
-        int grid = launch_blocks((const void *)k_pairs_valu<2, MODE>, ds->cu);
-        hipLaunchKernelGGL((k_pairs_valu<2, MODE>), dim3(grid), dim3(256), 0, s, a);
+        int grid = launch_blocks((const void *)k_pairs_valu<2, MODE>, cu);
+        hipLaunchKernelGGL((k_pairs_valu<2, MODE>), dim3(grid), dim3(256), 0, s, bp);
     }
     HIPCHK(hipGetLastError());
     return EPISDET_OK;
+}
+
+template <int MODE>
+int dispatch_pairs(episdet_dataset *ds, int tile, PairArgs &a, hipStream_t s)
+{
+    CHK(upload_single(ds, a, a.nunits));
+    BatchPair bp;
+    bp.jobs = (const PairArgs *)ds->jobargs;
+    bp.job_start = ds->jobstart;
+    bp.njobs = 1;
+    bp.total = a.nunits;
+    bp.queue = a.queue;
+    return launch_pairs_batch<MODE>(PAIR_TILES[tile], ds->cu, bp, s);
 }
 
 // Groups of 3-way units: (Y tile, Z tile) pairs that hold at least one
@@ -423,6 +460,41 @@ int rescore(episdet_dataset *ds, int order, int score, int count, double *ms)
 
 int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res);
 
+// Winner among the `count` re-scored candidates on the device (FP64 score,
+// then the lowest packed index), and the FP32-vs-FP64 error over them.
+int collect_best(episdet_dataset *ds, int order, int score, int count, episdet_result *res)
+{
+    std::vector<unsigned long long> hp(count);
+    std::vector<float> h32(count);
+    std::vector<double> h64(count);
+    HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h32.data(), ds->cand_s32, sizeof(float) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
+    double bs = score == EPISDET_SCORE_K2 ? INFINITY : -INFINITY;
+    unsigned long long bp = ~0ull;
+    float b32 = 0.f;
+    double maxabs = 0, maxrel = 0;
+    for (int i = 0; i < count; i++) {
+        const double s32 = score == EPISDET_SCORE_K2 ? (double)h32[i] : -(double)h32[i];
+        const double err = std::fabs(s32 - h64[i]);
+        maxabs = std::max(maxabs, err);
+        if (h64[i] != 0) maxrel = std::max(maxrel, err / std::fabs(h64[i]));
+        if (better(score, h64[i], hp[i], bs, bp)) {
+            bs = h64[i];
+            bp = hp[i];
+            b32 = (float)s32;
+        }
+    }
+    res->best_score = bs;
+    res->best_score_f32 = b32;
+    res->packed = bp;
+    unpack(order, bp, res->idx);
+    res->fp32_max_abs_err = maxabs;
+    res->fp32_max_rel_err = maxrel;
+    res->candidates = count;
+    return EPISDET_OK;
+}
+
 // Fused search (VALU): one persistent launch, then FP64 re-scoring.
 int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
 {
@@ -514,33 +586,7 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     double rms = 0;
     CHK(rescore(ds, order, score, count, &rms));
     res->rescore_ms = rms;
-    std::vector<unsigned long long> hp(count);
-    std::vector<float> h32(count);
-    std::vector<double> h64(count);
-    HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h32.data(), ds->cand_s32, sizeof(float) * count, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
-    double bs = score == EPISDET_SCORE_K2 ? INFINITY : -INFINITY;
-    unsigned long long bp = ~0ull;
-    float b32 = 0.f;
-    double maxabs = 0, maxrel = 0;
-    for (int i = 0; i < count; i++) {
-        const double s32 = score == EPISDET_SCORE_K2 ? (double)h32[i] : -(double)h32[i];
-        const double err = std::fabs(s32 - h64[i]);
-        maxabs = std::max(maxabs, err);
-        if (h64[i] != 0) maxrel = std::max(maxrel, err / std::fabs(h64[i]));
-        if (better(score, h64[i], hp[i], bs, bp)) {
-            bs = h64[i];
-            bp = hp[i];
-            b32 = (float)s32;
-        }
-    }
-    res->best_score = bs;
-    res->best_score_f32 = b32;
-    res->packed = bp;
-    unpack(order, bp, res->idx);
-    res->fp32_max_abs_err = maxabs;
-    res->fp32_max_rel_err = maxrel;
+    CHK(collect_best(ds, order, score, count, res));
     res->path = mfma ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
     return EPISDET_OK;
 }
@@ -670,6 +716,23 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
 
 long long sets_total(int order, int n) { return (long long)episdet_n_choose_k((unsigned)n, (unsigned)order); }
 
+struct DevList {   // device allocations released together
+    std::vector<void *> ptrs;
+    ~DevList()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    int put(T **dst, const T *src, size_t n)
+    {
+        if (hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return EPISDET_ERR_ALLOC;
+        ptrs.push_back((void *)*dst);
+        if (n && hipMemcpy((void *)*dst, (const void *)src, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess)
+            return EPISDET_ERR_HIP;
+        return EPISDET_OK;
+    }
+};
+
 }  // namespace
 
 // ================================================================ C ABI ==
@@ -754,6 +817,9 @@ int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int 
     int rc = EPISDET_OK;
     if (hipStreamCreateWithFlags(&ds->stream, hipStreamNonBlocking) != hipSuccess) rc = EPISDET_ERR_HIP;
     if (rc == EPISDET_OK && hipMalloc(&ds->ctl, sizeof(Ctl)) != hipSuccess) rc = EPISDET_ERR_ALLOC;
+    if (rc == EPISDET_OK && hipMalloc(&ds->jobargs, 1024) != hipSuccess) rc = EPISDET_ERR_ALLOC;
+    if (rc == EPISDET_OK && hipMalloc(&ds->jobstart, 2 * sizeof(unsigned long long)) != hipSuccess)
+        rc = EPISDET_ERR_ALLOC;
     for (int c = 0; c < 2 && rc == EPISDET_OK; c++) {
         const int Wv = (ds->N[c] + 31) / 32;
         ds->W[c] = ((Wv + MF_KW - 1) / MF_KW) * MF_KW;   // a multiple of every kernel's chunk
@@ -798,6 +864,8 @@ void episdet_dataset_destroy(episdet_dataset *ds)
     if (ds->singles) (void)hipFree(ds->singles);
     if (ds->pairs) (void)hipFree(ds->pairs);
     if (ds->ctl) (void)hipFree(ds->ctl);
+    if (ds->jobargs) (void)hipFree(ds->jobargs);
+    if (ds->jobstart) (void)hipFree(ds->jobstart);
     if (ds->cand) (void)hipFree(ds->cand);
     if (ds->cand_s32) (void)hipFree(ds->cand_s32);
     if (ds->cand_s64) (void)hipFree(ds->cand_s64);
@@ -863,6 +931,229 @@ int episdet_search(int order, int score, const unsigned *cases, const unsigned *
     *outputFromGpu = (float)r.best_score;
     *packedIndices = r.packed;
     return EPISDET_OK;
+}
+
+// ------------------------------------------------------ batched search --
+
+int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, int njobs, episdet_result *results,
+                      double *batch_ms)
+{
+    if (!dss || !opts || !results || njobs <= 0) return EPISDET_ERR_INVALID;
+    const double t0 = now_s();
+    std::vector<int> tri, pairs, solo;
+    std::vector<const episdet_dataset *> seen;   // datasets already in the graph
+    for (int j = 0; j < njobs; j++) {
+        episdet_dataset *ds = dss[j];
+        const episdet_options &o = opts[j];
+        std::memset(&results[j], 0, sizeof(episdet_result));
+        if (!ds || (o.order != 2 && o.order != 3) || (o.score != EPISDET_SCORE_K2 && o.score != EPISDET_SCORE_MI) ||
+            ds->n < o.order)
+            return EPISDET_ERR_INVALID;
+        const double tp = now_s();
+        if (o.rebuild) {
+            CHK(compute_singles(ds));
+            if (ds->pairs) {
+                (void)hipFree(ds->pairs);
+                ds->pairs = nullptr;
+            }
+        }
+        CHK(ensure_tables(ds, o.score));
+        if (o.order == 3) CHK(ensure_pairs(ds));
+        CHK(ensure_cand(ds, o.candidate_cap > 0 ? (unsigned)o.candidate_cap : DEFAULT_CAP));
+        results[j].prep_ms = (now_s() - tp) * 1e3;
+        // 3-way jobs go to the batched FP4 MFMA kernel, 2-way jobs to the
+        // batched VALU kernel; anything else runs on its own afterwards.
+        // A dataset's control block and candidate buffer hold one search at a
+        // time, so a second job on the same dataset runs after the graph.
+        const bool dup = std::find(seen.begin(), seen.end(), ds) != seen.end();
+        if (dup || o.path == EPISDET_PATH_PLAIN || (o.order == 3 && !mfma_ok(ds))) {
+            solo.push_back(j);
+            continue;
+        }
+        seen.push_back(ds);
+        if (o.order == 3) tri.push_back(j);
+        else pairs.push_back(j);
+    }
+    DevList dl;
+    int cu = dss[0]->cu;
+    // per-job kernel arguments
+    std::vector<TriArgs> ta(tri.size());
+    std::vector<unsigned long long> tstart(1, 0ull);
+    for (size_t k = 0; k < tri.size(); k++) {
+        episdet_dataset *ds = dss[tri[k]];
+        std::vector<int2> groups;
+        std::vector<unsigned long long> gstart;
+        build_groups(ds->n, M2_TY, M2_TZ, groups, gstart);
+        unsigned long long *d_gstart;
+        int2 *d_groups;
+        CHK(dl.put(&d_gstart, gstart.data(), gstart.size()));
+        CHK(dl.put(&d_groups, groups.data(), groups.size()));
+        fill_tri_common(ds, ta[k]);
+        ta[k].sc = make_ctx(ds, opts[tri[k]].score, 3);
+        ta[k].gstart = d_gstart;
+        ta[k].groups = d_groups;
+        ta[k].ngroups = (int)groups.size();
+        ta[k].nunits = gstart.back();
+        tstart.push_back(tstart.back() + gstart.back());
+    }
+    std::vector<PairArgs> pa(pairs.size());
+    std::vector<unsigned long long> pstart(1, 0ull);
+    for (size_t k = 0; k < pairs.size(); k++) {
+        episdet_dataset *ds = dss[pairs[k]];
+        fill_pair_common(ds, pa[k], 16 * PAIR_TILES[0]);
+        pa[k].sc = make_ctx(ds, opts[pairs[k]].score, 2);
+        pstart.push_back(pstart.back() + pa[k].nunits);
+    }
+    std::vector<int> batched(tri);
+    batched.insert(batched.end(), pairs.begin(), pairs.end());
+    std::vector<RescoreJob> rj(batched.size());
+    std::vector<unsigned long long *> qp(batched.size());
+    std::vector<unsigned *> bp(batched.size()), cp(batched.size());
+    for (size_t k = 0; k < batched.size(); k++) {
+        episdet_dataset *ds = dss[batched[k]];
+        RescoreArgs &r = rj[k].r;
+        std::memset(&r, 0, sizeof r);
+        r.p0 = ds->planes[0];
+        r.p1 = ds->planes[1];
+        r.W0 = ds->W[0];
+        r.W1 = ds->W[1];
+        r.n = ds->n;
+        r.N0 = ds->N[0];
+        r.N1 = ds->N[1];
+        r.order = opts[batched[k]].order;
+        r.score = opts[batched[k]].score;
+        r.singles = ds->singles;
+        r.pairs = ds->pairs;
+        r.t64 = ds->tab[r.score].t64;
+        r.cand = ds->cand;
+        r.out_s64 = ds->cand_s64;
+        rj[k].count = &ds->ctl->cand_count;
+        rj[k].cap = ds->cand_cap;
+        qp[k] = &ds->ctl->queue;
+        bp[k] = &ds->ctl->best32;
+        cp[k] = &ds->ctl->cand_count;
+    }
+    TriArgs *d_ta = nullptr;
+    PairArgs *d_pa = nullptr;
+    unsigned long long *d_tstart = nullptr, *d_pstart = nullptr, *d_bq = nullptr;
+    RescoreJob *d_rj = nullptr;
+    unsigned long long **d_qp = nullptr;
+    unsigned **d_bp = nullptr, **d_cp = nullptr;
+    const unsigned long long zero2[2] = {0ull, 0ull};
+    CHK(dl.put(&d_ta, ta.data(), ta.size()));
+    CHK(dl.put(&d_pa, pa.data(), pa.size()));
+    CHK(dl.put(&d_tstart, tstart.data(), tstart.size()));
+    CHK(dl.put(&d_pstart, pstart.data(), pstart.size()));
+    CHK(dl.put(&d_bq, zero2, 2));
+    CHK(dl.put(&d_rj, rj.data(), rj.size()));
+    CHK(dl.put(&d_qp, qp.data(), qp.size()));
+    CHK(dl.put(&d_bp, bp.data(), bp.size()));
+    CHK(dl.put(&d_cp, cp.data(), cp.size()));
+
+    double gms = 0;
+    if (!batched.empty()) {
+        int per3 = 0, per2 = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per3, (const void *)k_triplets_mfma2<MODE_SEARCH>, 512, 0) !=
+                hipSuccess ||
+            per3 < 1)
+            per3 = 1;
+        per2 = launch_blocks((const void *)k_pairs_valu<4, MODE_SEARCH>, cu) / cu;
+        hipStream_t st;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+        // one graph: reset, 3-way batch, 2-way batch, FP64 re-scoring
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int nb = (int)batched.size();
+        hipLaunchKernelGGL(k_init_many, dim3((std::max(nb, 2) + 63) / 64), dim3(64), 0, st, d_qp, d_bp, d_cp, nb, d_bq,
+                           2);
+        if (!tri.empty()) {
+            BatchTri bt;
+            bt.jobs = d_ta;
+            bt.job_start = d_tstart;
+            bt.njobs = (int)tri.size();
+            bt.total = tstart.back();
+            bt.queue = d_bq;
+            hipLaunchKernelGGL((k_triplets_mfma2<MODE_SEARCH>), dim3(per3 * cu), dim3(512), 0, st, bt);
+        }
+        if (!pairs.empty()) {
+            BatchPair bpr;
+            bpr.jobs = d_pa;
+            bpr.job_start = d_pstart;
+            bpr.njobs = (int)pairs.size();
+            bpr.total = pstart.back();
+            bpr.queue = d_bq + 1;
+            hipLaunchKernelGGL((k_pairs_valu<4, MODE_SEARCH>), dim3(per2 * cu), dim3(256), 0, st, bpr);
+        }
+        hipLaunchKernelGGL(k_rescore_batch, dim3(8 * cu), dim3(64), 0, st, d_rj, nb);
+        HIPCHK(hipStreamEndCapture(st, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, st));
+        HIPCHK(hipGraphLaunch(exec, st));
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float f = 0;
+        HIPCHK(hipEventElapsedTime(&f, e0, e1));
+        gms = f;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        (void)hipStreamDestroy(st);
+    }
+    for (size_t k = 0; k < batched.size(); k++) {
+        const int j = batched[k];
+        episdet_dataset *ds = dss[j];
+        Ctl hc;
+        HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
+        if (hc.cand_count == 0 || hc.cand_count > ds->cand_cap) {
+            solo.push_back(j);     // overflow: exact rerun on its own
+            continue;
+        }
+        episdet_result &r = results[j];
+        CHK(collect_best(ds, opts[j].order, opts[j].score, (int)hc.cand_count, &r));
+        r.kernel_ms = gms;
+        r.margin = make_ctx(ds, opts[j].score, opts[j].order).margin;
+        r.path = opts[j].order == 3 ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
+        r.rounds = episdet_reference_rounds(opts[j].order, ds->n);
+        r.sets = sets_total(opts[j].order, ds->n);
+    }
+    for (int j : solo) {
+        const double pm = results[j].prep_ms;
+        episdet_options o = opts[j];
+        o.rebuild = 0;
+        if (o.path != EPISDET_PATH_PLAIN) o.path = EPISDET_PATH_AUTO;
+        CHK(episdet_run(dss[j], &o, &results[j]));
+        results[j].prep_ms = pm;
+    }
+    const double wall = now_s() - t0;
+    for (int j = 0; j < njobs; j++) results[j].wall_s = wall;
+    if (batch_ms) *batch_ms = gms;
+    return EPISDET_OK;
+}
+
+int episdet_search_batch(const episdet_job *jobs, int njobs, const episdet_options *opt, episdet_result *results,
+                         double *batch_ms)
+{
+    if (!jobs || njobs <= 0 || !results) return EPISDET_ERR_INVALID;
+    std::vector<episdet_dataset *> dss(njobs, nullptr);
+    std::vector<episdet_options> opts(njobs);
+    int rc = EPISDET_OK;
+    for (int j = 0; j < njobs && rc == EPISDET_OK; j++) {
+        const episdet_job &jb = jobs[j];
+        rc = episdet_dataset_create(jb.cases, jb.controls, jb.numSNPs, jb.numCases, jb.numControls, jb.casesWords,
+                                    jb.controlsWords, &dss[j]);
+        std::memset(&opts[j], 0, sizeof(episdet_options));
+        if (opt) opts[j] = *opt;
+        opts[j].order = jb.order;
+        opts[j].score = jb.score;
+    }
+    if (rc == EPISDET_OK) rc = episdet_run_batch(dss.data(), opts.data(), njobs, results, batch_ms);
+    for (episdet_dataset *ds : dss) episdet_dataset_destroy(ds);
+    return rc;
 }
 
 int episdet_tables(episdet_dataset *ds, int order, const int *sets, int m, unsigned *out)

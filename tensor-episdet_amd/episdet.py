@@ -76,14 +76,21 @@ class GenParams(C.Structure):
                 ("planted_maf_lo", C.c_double), ("planted_maf_hi", C.c_double), ("threads", C.c_int)]
 
 
+class Job(C.Structure):
+    _fields_ = [("order", C.c_int), ("score", C.c_int), ("cases", C.c_void_p), ("controls", C.c_void_p),
+                ("numSNPs", C.c_int), ("numCases", C.c_int), ("numControls", C.c_int),
+                ("casesWords", C.c_int), ("controlsWords", C.c_int)]
+
+
 _u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 # Every symbol include/episdet.h and include/episdet_data.h declare.
 SEARCH_SYMBOLS = ("episdet_search", "episdet_dataset_create", "episdet_dataset_destroy", "episdet_run",
-                  "episdet_tables", "episdet_dump_tables", "episdet_checksum", "episdet_n_choose_k",
-                  "episdet_reference_rounds", "episdet_status_string", "episdet_device_info")
+                  "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
+                  "episdet_checksum", "episdet_n_choose_k", "episdet_reference_rounds", "episdet_status_string",
+                  "episdet_device_info")
 DATA_SYMBOLS = ("episdet_gen_defaults", "episdet_generate", "episdet_padded_words", "episdet_write_dataset",
                 "episdet_read_header", "episdet_read_planes")
 
@@ -126,6 +133,12 @@ def search_lib():
         lib.episdet_dataset_destroy.restype = None
         lib.episdet_run.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result)]
         lib.episdet_run.restype = C.c_int
+        lib.episdet_run_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Options), C.c_int, C.POINTER(Result),
+                                          C.POINTER(C.c_double)]
+        lib.episdet_run_batch.restype = C.c_int
+        lib.episdet_search_batch.argtypes = [C.POINTER(Job), C.c_int, C.POINTER(Options), C.POINTER(Result),
+                                             C.POINTER(C.c_double)]
+        lib.episdet_search_batch.restype = C.c_int
         lib.episdet_tables.argtypes = [C.c_void_p, C.c_int, _i32p, C.c_int, _u32p]
         lib.episdet_tables.restype = C.c_int
         lib.episdet_dump_tables.argtypes = [C.c_void_p, C.c_int, C.c_int, _u32p, C.c_size_t]
@@ -246,6 +259,42 @@ def search(order, score, cases, controls, num_cases, num_controls):
     if rc:
         raise EpisdetError(rc, "episdet_search")
     return packed.value, best.value, rounds.value, secs.value
+
+
+def run_batch(datasets, orders, scores, rebuild=False, path=PATH_AUTO):
+    """Many resident searches in one graph launch; returns (results, batch_ms)."""
+    n = len(datasets)
+    hs = (C.c_void_p * n)(*[d._h for d in datasets])
+    opts = (Options * n)()
+    for j in range(n):
+        opts[j].order, opts[j].score, opts[j].path = orders[j], scores[j], path
+        opts[j].rebuild = 1 if rebuild else 0
+    res = (Result * n)()
+    ms = C.c_double()
+    rc = search_lib().episdet_run_batch(hs, opts, n, res, C.byref(ms))
+    if rc:
+        raise EpisdetError(rc, "episdet_run_batch")
+    return [r.as_dict() for r in res], ms.value
+
+
+def search_batch(jobs):
+    """jobs: list of (order, score, Synthetic).  Host buffers, one call;
+    returns (results, batch_ms)."""
+    n = len(jobs)
+    arr = (Job * n)()
+    keep = []
+    for j, (order, score, syn) in enumerate(jobs):
+        cs = np.ascontiguousarray(syn.cases, dtype=np.uint32)
+        ct = np.ascontiguousarray(syn.controls, dtype=np.uint32)
+        keep += [cs, ct]
+        arr[j] = Job(order, score, cs.ctypes.data, ct.ctypes.data, cs.shape[0], syn.num_cases, syn.num_controls,
+                     cs.shape[2], ct.shape[2])
+    res = (Result * n)()
+    ms = C.c_double()
+    rc = search_lib().episdet_search_batch(arr, n, None, res, C.byref(ms))
+    if rc:
+        raise EpisdetError(rc, "episdet_search_batch")
+    return [r.as_dict() for r in res], ms.value
 
 
 def unpack(order, packed):

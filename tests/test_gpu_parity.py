@@ -231,3 +231,40 @@ def test_mi_pairs_uneven_classes_planted():
     with episdet.Dataset.from_synthetic(syn) as ds:
         r = check_best(ds, syn.cases, syn.controls, 26214, 39322, 2, episdet.MI)
         assert tuple(r["idx"][:2]) == syn.planted[0]
+
+
+# ------------------------------------------------------------- batches --
+
+def test_batch_of_uneven_jobs_matches_oracle():
+    """Mixed 2-/3-way, K2/MI jobs of uneven sizes and tight (non-1024)
+    padding in one graph launch: every winner and FP64 score equals the
+    oracle's."""
+    rng = np.random.default_rng(4)
+    jobs = []
+    for j in range(12):
+        order = 3 if j % 2 else 2
+        n = int(rng.integers(40, 120 if order == 3 else 300))
+        Nc, Nt = int(rng.integers(300, 5000)), int(rng.integers(300, 5000))
+        syn = episdet.generate(n, Nc, Nt, seed=100 + j, cases_words=(Nc + 31) // 32 + (j % 3),
+                               controls_words=(Nt + 31) // 32, planted_sets=1 if j % 3 == 0 else 0,
+                               planted_order=order)
+        jobs.append((order, episdet.MI if j % 4 >= 2 else episdet.K2, syn))
+    say("batch of 12 via host buffers")
+    res, ms = episdet.search_batch(jobs)
+    assert ms > 0
+    for (order, score, syn), r in zip(jobs, res):
+        o = O.search(syn.cases, syn.controls, syn.num_cases, syn.num_controls, order, score)
+        assert r["packed"] == o["packed"], (order, score, syn.num_snps, r, o)
+        assert r["best_score"] == o["score"]
+    say("same batch resident, twice (graph relaunch)")
+    dss = [episdet.Dataset.from_synthetic(syn) for _, _, syn in jobs]
+    for rep in range(2):
+        res2, _ = episdet.run_batch(dss, [o for o, _, _ in jobs], [s for _, s, _ in jobs], rebuild=rep == 1)
+        assert [r["packed"] for r in res2] == [r["packed"] for r in res]
+    say("one dataset in two jobs of one batch")
+    res3, _ = episdet.run_batch([dss[1], dss[1], dss[0]], [3, 2, 2], [episdet.K2, episdet.MI, episdet.K2])
+    assert res3[0]["packed"] == res[1]["packed"] and res3[2]["packed"] == res[0]["packed"]
+    o = O.search(jobs[1][2].cases, jobs[1][2].controls, jobs[1][2].num_cases, jobs[1][2].num_controls, 2, episdet.MI)
+    assert res3[1]["packed"] == o["packed"] and res3[1]["best_score"] == o["score"]
+    for d in dss:
+        d.close()
