@@ -24,6 +24,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3|5] [--path m
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -54,6 +55,22 @@ PRESETS = {
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic(n, Nc, Nt, order, score, path, tile):
+    """HBM bytes per launch of the dominant kernel for this exact workload, as
+    recorded from the FETCH_SIZE / WRITE_SIZE counter passes (calibrated
+    against the copy kernel) in profiles/traffic.json; None if that
+    workload was not profiled."""
+    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    if not os.path.exists(f):
+        return None
+    with open(f) as fh:
+        for e in json.load(fh)["entries"]:
+            if (e["snps"], e["cases"], e["controls"], e["order"], e["score"], e["path"], e["tile"]) == \
+                    (n, Nc, Nt, order, SCORE_NAME[score], path, tile):
+                return e["bytes"]
+    return None
 
 
 def batch_jobs(seed):
@@ -158,8 +175,6 @@ def main():
     Nc = int(round(N * pre["case_frac"]))
     Nt = N - Nc
     order, score = pre["order"], pre["score"]
-    if order == 2:
-        args.path = "valu"
     t = time.time()
     syn = episdet.generate(n, Nc, Nt, seed=args.seed, planted_sets=pre["planted"], planted_order=order,
                            maf=pre["maf"], maf_dist=pre["maf_dist"])
@@ -193,13 +208,14 @@ def main():
     if args.path == "mfma":
         achieved = FLOPS_PER_SET_SAMPLE[order] * sets * N / kernel_s
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_FP4_PEAK / 1e12,
-                "unit": "TFLOP/s (dense FP4 MFMA, 16 flop per set-sample)", "frac": achieved / MFMA_FP4_PEAK}
+                "unit": f"TFLOP/s (dense FP4 MFMA, {FLOPS_PER_SET_SAMPLE[order]:.0f} flop per set-sample)", "frac": achieved / MFMA_FP4_PEAK}
     else:
         achieved = OPS_PER_SET_SAMPLE[order] * sets * N / kernel_s
         roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12,
-                "unit": "T lane-ops/s (32-bit VALU, 0.5 per set-sample)", "frac": achieved / VALU_PEAK_OPS}
-    roof.update({"traffic": None, "kernel_ms": kernel_s * 1e3})
+                "unit": f"T lane-ops/s (32-bit VALU, {OPS_PER_SET_SAMPLE[order]} per set-sample)", "frac": achieved / VALU_PEAK_OPS}
     r = results[-1]
+    roof.update({"traffic": measured_traffic(n, Nc, Nt, order, score, "fused_" + args.path, r["tile"]),
+                 "kernel_ms": kernel_s * 1e3})
     planted_ok = (tuple(r["idx"][:order]) in syn.planted) if syn.planted else None
 
     plain = None
@@ -216,11 +232,15 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
         threads = min(os.cpu_count() or 1, 16)
-        # calibrate a slice of the search (largest index in [z0, z0+zn)) to
-        # about --cpu-seconds: the marginal cost of one more Z from two probes
-        # (each probe also pays for its pair table, which cancels out)
-        # The oracle parallelises over the largest index, so each probe spans
-        # a multiple of the thread count.
+        # Calibrate a slice of the search (the sets whose largest index is in
+        # [z0, z0+zn)) to about --cpu-seconds.  A set's cost is the same
+        # everywhere, and C(Z, order-1) sets have largest index Z, so two
+        # probes give the marginal cost per set (each probe also pays for its
+        # own setup, which cancels out).  The oracle parallelises over the
+        # largest index, so each probe spans a multiple of the thread count.
+        def sets_in(lo, hi):
+            return sum(math.comb(z, order - 1) for z in range(lo, hi))
+
         z0 = max(order, min(n // 4, 600))
         probe_t = []
         ks = (threads, 3 * threads)
@@ -229,8 +249,13 @@ def main():
             oracle.search(syn.cases, syn.controls, Nc, Nt, order, score, threads=threads,
                           last=(z0, min(n, z0 + k)))
             probe_t.append(time.perf_counter() - t0)
-        per_z = max((probe_t[1] - probe_t[0]) / (ks[1] - ks[0]), probe_t[0] / ks[0] / 4, 1e-4)
-        zn = int(max(threads, min(n - z0, 4 * threads * 64, args.cpu_seconds / per_z)))
+        s1, s2 = sets_in(z0, min(n, z0 + ks[0])), sets_in(z0, min(n, z0 + ks[1]))
+        per_set = max((probe_t[1] - probe_t[0]) / max(s2 - s1, 1), probe_t[0] / max(s1, 1) / 4, 1e-12)
+        target = args.cpu_seconds / per_set
+        zn, acc = 0, 0
+        while z0 + zn < n and (zn < threads or acc < target):
+            acc += math.comb(z0 + zn, order - 1)
+            zn += 1
         log(f"cpu: probes {probe_t[0]:.2f} s / {probe_t[1]:.2f} s, timing largest index in [{z0},{z0 + zn})")
         t0 = time.perf_counter()
         c = oracle.search(syn.cases, syn.controls, Nc, Nt, order, score, threads=threads, last=(z0, z0 + zn))

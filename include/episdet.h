@@ -98,7 +98,9 @@ typedef struct {
                              this call, so their cost is part of it (the
                              reference counts them inside its search,
                              search-triplets.cu:113-117,173-197)                 */
-    int reserved[7];
+    int topk;             /* > 1: keep the topk best sets (episdet_run_topk);
+                             0 or 1: the single best set                         */
+    int reserved[6];
 } episdet_options;
 
 typedef struct {
@@ -128,6 +130,19 @@ typedef struct {
 } episdet_result;
 
 int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result *res);
+
+/* The k best sets of one search, by exact FP64 score: K2 ascending or MI
+ * descending, ties to the lowest packed index.  The reference keeps only the
+ * single best set (helper.cu:552-583, reduction.hpp:64-122); this widens the
+ * same reduction to k.  One fused launch keeps every set whose FP32 score is
+ * within the error margin of a running bound on the k-th best; those are
+ * re-scored in FP64 and ranked.  packed / scores: capacity k, best first;
+ * *count receives min(k, C(n, order)); res as for episdet_run (the best set).
+ * 1 <= k <= EPISDET_TOPK_MAX.  Fused paths only (AUTO, FUSED_VALU,
+ * FUSED_MFMA): PLAIN returns EPISDET_ERR_INVALID. */
+#define EPISDET_TOPK_MAX 1024
+int episdet_run_topk(episdet_dataset *ds, const episdet_options *opt, int k, unsigned long long *packed,
+                     double *scores, int *count, episdet_result *res);
 
 /* ---- batched searches (config "many independent jobs", no reference
  * counterpart: the reference runs one search per process) ---- */

@@ -7,6 +7,7 @@ imports it.
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import os
 
 import numpy as np
@@ -158,6 +159,29 @@ def all_tables(cases, controls, Nc, Nt, order, want_tables=True):
     ptr = out.ctypes.data_as(C.c_void_p) if want_tables else None
     cs = lib().orc_all_tables(cases, controls, n, Nc, Nt, cases.shape[2], controls.shape[2], order, ptr)
     return out, int(cs)
+
+
+def ranked(cases, controls, Nc, Nt, order, score):
+    """Every set of a small dataset ranked best first by its FP64 score
+    (K2 ascending, MI descending; ties to the lowest packed index, the tie
+    rule of orc_search).  Returns (packed[], scores[]) as numpy arrays.  The
+    reference keeps only the first entry (helper.cu:552-583)."""
+    tabs, _ = all_tables(cases, controls, Nc, Nt, order)
+    n = cases.shape[0]
+    N = Nc + Nt
+    if score == 0:
+        t = lnfact_table(N + 1)
+        sc = np.array([score_k2(c[0], c[1], t) for c in tabs])
+    else:
+        t = log2_table(N)
+        sc = np.array([score_mi(c[0], c[1], t) for c in tabs])
+    sets = list(itertools.combinations(range(n), order))   # colex rank order below
+    sets.sort(key=lambda s: s[::-1])
+    packed = np.array([s[0] | (s[1] << 21) | (s[2] << 42) if order == 3 else s[0] | (s[1] << 32) for s in sets],
+                      dtype=np.uint64)
+    key = sc if score == 0 else -sc
+    o = np.lexsort((packed, key))
+    return packed[o], sc[o]
 
 
 def uses_avx512():

@@ -247,7 +247,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(TriArgs p)
             const Singles sy = p.singles[Y];
             uint32_t cas[27], ctl[27];
             triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
-            if constexpr (MODE == MODE_SEARCH) {
+            if constexpr (scores_sets(MODE)) {
                 bad[q4] = badness_f32(p.sc, cas, ctl);
             } else if constexpr (MODE == MODE_DUMP) {
                 uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
@@ -260,6 +260,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(TriArgs p)
             }
         }
         if constexpr (MODE == MODE_SEARCH) publish_candidates<4, 8>(p.co, p.sc.margin, bad, packed, s_red);
+        if constexpr (MODE == MODE_TOPK) publish_topk<4, 8>(p.co, p.sc.margin, bad, packed);
         if constexpr (MODE == MODE_CHECKSUM) {
             if (hsum) atomicAdd(p.checksum, hsum);
         }
@@ -492,7 +493,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
                 const Singles sy = p.singles[Y];
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
-                if constexpr (MODE == MODE_SEARCH) {
+                if constexpr (scores_sets(MODE)) {
                     bad[m] = badness_f32(p.sc, cas, ctl);
                 } else if constexpr (MODE == MODE_DUMP) {
                     uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
@@ -507,6 +508,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
             __syncthreads();
         }
         if constexpr (MODE == MODE_SEARCH) publish_candidates<8, 8>(p.co, p.sc.margin, bad, packed, s_red);
+        if constexpr (MODE == MODE_TOPK) publish_topk<8, 8>(p.co, p.sc.margin, bad, packed);
         if constexpr (MODE == MODE_CHECKSUM) {
             if (hsum) atomicAdd(p.checksum, hsum);
         }

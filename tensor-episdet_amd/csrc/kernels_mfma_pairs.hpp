@@ -1,0 +1,250 @@
+// kernels_mfma_pairs.hpp -- fused 2-way search counting on the matrix cores.
+//
+// The four AND-counts of every pair (A, B) are one dense contraction over
+// samples:
+//   C[(A,i), (B,j)] = sum_s A_i(s) * B_j(s)
+// a GEMM with rows (A, plane i), columns (B, plane j) and K = samples.  It
+// runs on v_mfma_scale_f32_32x32x64_f8f6f4 with the FP4 encoding of the
+// 3-way kernel (class-split nibble masks, E8M0 scales that make every
+// product exactly 1.0, exact FP32 accumulation below 2^24 samples per
+// class); here the A operand is a plane row itself, with no X & Y product.
+// The reference builds these counts as a CUTLASS b1 XOR-popcount GEMM plus
+// a recovery step (helper.cu:714-750, 636-637); its pair tables follow
+// helper.cu:644-662.
+//
+// Work unit: 128 A SNPs x 64 B SNPs (A tile tA, B tile tB, tA <= tB / 2 so
+// that some A < B exists).  512 threads = 8 waves: waves 0-3 contract the
+// cases, 4-7 the controls; wave (class, wy) owns A rows 64wy..64wy+63 (32
+// SNPs x 2 planes) x all 128 B rows = 2 x 4 MFMA tiles, 128 accumulators, so
+// each A operand feeds four MFMAs and each B operand two.  The rows of a
+// 512-sample chunk (256 A rows + 128 B rows, 64 bytes each) go straight from
+// global memory to LDS (global_load_lds_dwordx4), double-buffered; the four
+// 16-byte slots of row r are XOR-swizzled by (r >> 1) & 3, so the eight
+// consecutive rows a group of lanes reads at one logical slot cover all
+// banks once.  After the last chunk the accumulators go to LDS in two
+// halves (B 0-31, then B 32-63) and all 512 threads build and score eight
+// pair tables per half.
+#pragma once
+#include "kernels_mfma.hpp"
+
+namespace episdet {
+
+constexpr int P2_TA = 128, P2_TB = 64;
+constexpr int P2_KW = 16;                                   // words per chunk (512 samples)
+constexpr int P2_AROWS = 2 * P2_TA;                         // 256
+constexpr int P2_ROWS = P2_AROWS + 2 * P2_TB;               // 384 rows of a class
+constexpr int P2_ROWS_PER_LOAD = 64 * 4 / P2_KW;            // 16: one load instruction = 1 KB
+constexpr int P2_LOADS = P2_ROWS / P2_ROWS_PER_LOAD;        // 24 per class and chunk
+constexpr int P2_BUF_WORDS = P2_ROWS * P2_KW;               // 6144 (24 KB)
+constexpr int P2_STAGE_WORDS = 4 * P2_BUF_WORDS;            // 2 classes x 2 buffers (96 KB)
+constexpr int P2_DUMP_WORDS = 2 * 4 * 4 * 16 * 64;          // class x wave x tile x reg x lane (128 KB)
+constexpr int P2_SMEM_WORDS = P2_STAGE_WORDS > P2_DUMP_WORDS ? P2_STAGE_WORDS : P2_DUMP_WORDS;
+static_assert(P2_LOADS % 4 == 0, "loads split over the four waves of a class");
+static_assert(P2_ROWS_PER_LOAD == 16 && P2_KW == 16, "swizzle assumes 4 slots per row");
+
+// Units of the B-tile-major enumeration: B tile tB holds floor(tB/2)+1 units
+// (tA = 0..floor(tB/2)), so units before B tile 2m are m(m+1) and before
+// 2m+1 are (m+1)^2.
+__host__ __device__ inline unsigned long long p2_units(int ntb)
+{
+    const unsigned long long m = (unsigned long long)(ntb / 2);
+    return (ntb & 1) ? (m + 1) * (m + 1) : m * (m + 1);
+}
+
+__device__ __forceinline__ void p2_unrank(unsigned long long u, int &ta, int &tb)
+{
+    long long m = (long long)((sqrt(4.0 * (double)u + 1.0) - 1.0) * 0.5);
+    while ((unsigned long long)(m + 1) * (unsigned long long)(m + 2) <= u) m++;
+    while ((unsigned long long)m * (unsigned long long)(m + 1) > u) m--;
+    const unsigned long long e = (unsigned long long)m * (unsigned long long)(m + 1);
+    if (u < e + (unsigned long long)(m + 1)) {
+        tb = (int)(2 * m);
+        ta = (int)(u - e);
+    } else {
+        tb = (int)(2 * m + 1);
+        ta = (int)(u - (unsigned long long)(m + 1) * (unsigned long long)(m + 1));
+    }
+}
+
+// Global row of LDS row r: A rows 2a+i, then B rows 2b+j.
+__device__ __forceinline__ const uint32_t *p2_rowptr(const uint32_t *base, int W, int a0, int b0, int r)
+{
+    if (r < P2_AROWS) return base + ((size_t)a0 * 2 + r) * W;
+    return base + ((size_t)b0 * 2 + (r - P2_AROWS)) * W;
+}
+
+// This wave's share of the direct-to-LDS loads of chunk w0 into buf.  One
+// instruction fills sixteen rows: lane L writes physical slot L%4 of row
+// L/4 of the block, which holds logical slot (L%4) ^ ((L/4 >> 1) & 3).
+__device__ __forceinline__ void p2_issue(const uint32_t *base, int W, int a0, int b0, int w0, uint32_t *buf,
+                                         int wih, int lane)
+{
+    const int rl = lane >> 2, q = (lane & 3) ^ ((rl >> 1) & 3);
+#pragma unroll
+    for (int ii = wih; ii < P2_LOADS; ii += 4) {
+        const uint32_t *src = p2_rowptr(base, W, a0, b0, ii * P2_ROWS_PER_LOAD + rl) + w0 + q * 4;
+        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)(buf + ii * P2_ROWS_PER_LOAD * P2_KW), 16,
+                                         0, 0);
+    }
+}
+
+// Word offset of logical 16-byte slot q of LDS row r.
+__device__ __forceinline__ int p2_at(int r, int q) { return r * P2_KW + ((q ^ ((r >> 1) & 3)) << 2); }
+
+template <int C>
+__device__ __forceinline__ void p2_step(const uint4 (&aw)[2], const uint4 (&bw)[4], v16f_t (&acc)[2][4])
+{
+    constexpr int s = mf_scale<C>();
+    const v8i_t A0 = mf_operand<C>(aw[0]), A1 = mf_operand<C>(aw[1]);
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) {
+        const v8i_t B = mf_operand<C>(bw[cb]);
+        acc[0][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A0, B, acc[0][cb], 4, 4, 0, s, 0, s);
+        acc[1][cb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A1, B, acc[1][cb], 4, 4, 0, s, 0, s);
+    }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
+{
+    __shared__ __attribute__((aligned(16))) uint32_t smem[P2_SMEM_WORDS];
+    __shared__ unsigned long long s_u;
+    __shared__ float s_red[16];
+    const int tid = threadIdx.x;
+    const int cls = tid >> 8, t = tid & 255;
+    const int wy = t >> 6, lane = t & 63, h = lane >> 5, i = lane & 31;
+    uint32_t *buf0 = smem + (cls * 2 + 0) * P2_BUF_WORDS;
+    uint32_t *buf1 = smem + (cls * 2 + 1) * P2_BUF_WORDS;
+    // LDS rows read by this lane: A rows of its two row tiles, B rows of the
+    // four column tiles (tile row / column i = 2 * snp + plane)
+    const int ar0 = 64 * wy + i, ar1 = 64 * wy + 32 + i;
+    int br[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) br[cb] = P2_AROWS + 32 * cb + i;
+
+    int j = 0;
+    for (;;) {
+        if (tid == 0) s_u = atomicAdd(bp.queue, 1ull);
+        __syncthreads();
+        const unsigned long long uu = s_u;
+        if (uu >= bp.total) break;
+        while (j + 1 < bp.njobs && bp.job_start[j + 1] <= uu) j++;
+        const PairArgs &p = bp.jobs[j];
+        const unsigned long long u = uu - bp.job_start[j];
+        int tA, tB;
+        p2_unrank(u, tA, tB);
+        const int a0 = tA * P2_TA, b0 = tB * P2_TB;
+        const uint32_t *base = cls ? p.p1 : p.p0;
+        const int W = cls ? p.W1 : p.W0;
+        const int nch_mine = W / P2_KW;
+        const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / P2_KW;
+
+        v16f_t acc[2][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
+
+        if (nch_mine > 0) p2_issue(base, W, a0, b0, 0, buf0, wy, lane);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        for (int c = 0; c < nch; c++) {
+            uint32_t *cur = (c & 1) ? buf1 : buf0;
+            if (c + 1 < nch_mine) p2_issue(base, W, a0, b0, (c + 1) * P2_KW, (c & 1) ? buf0 : buf1, wy, lane);
+            if (c < nch_mine) {
+#pragma unroll
+                for (int s = 0; s < P2_KW / 8; s++) {
+                    const int q = 2 * s + h;
+                    uint4 aw[2], bw[4];
+                    aw[0] = *(const uint4 *)&cur[p2_at(ar0, q)];
+                    aw[1] = *(const uint4 *)&cur[p2_at(ar1, q)];
+#pragma unroll
+                    for (int cb = 0; cb < 4; cb++) bw[cb] = *(const uint4 *)&cur[p2_at(br[cb], q)];
+                    p2_step<0>(aw, bw, acc);
+                    p2_step<1>(aw, bw, acc);
+                    p2_step<2>(aw, bw, acc);
+                    p2_step<3>(aw, bw, acc);
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+        }
+
+        float bad[16];
+        unsigned long long packed[16];
+        unsigned long long hsum = 0;
+#pragma unroll
+        for (int hf = 0; hf < 2; hf++) {
+            // tiles with column blocks 2hf, 2hf+1 -> LDS [cls][wy][t4][reg][lane]
+#pragma unroll
+            for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+                for (int cr = 0; cr < 2; cr++)
+#pragma unroll
+                    for (int r = 0; r < 16; r++)
+                        smem[((((cls * 4 + wy) * 4 + rb * 2 + cr) * 16) + r) * 64 + lane] =
+                            (uint32_t)acc[rb][2 * hf + cr][r];
+            __syncthreads();
+#pragma unroll
+            for (int q8 = 0; q8 < 8; q8++) {
+                const int m = hf * 8 + q8;
+                const int T = q8 * 512 + tid;                   // 128 A x 32 B of this half
+                const int al = T >> 5, bl = hf * 32 + (T & 31);
+                const int A = a0 + al, B = b0 + bl;
+                bad[m] = INFINITY;
+                packed[m] = (unsigned long long)A | ((unsigned long long)B << 32);
+                if (!(A < B && B < p.n)) continue;
+                // row 2*al + pi lives in wave al/32, row tile (al/16)%2, tile
+                // row rho = 2*(al%16) + pi; column 2*bl' + pj in column tile
+                // bl/16, tile column 2*(bl%16) + pj.  Output register
+                // r = 4*(rho/8) + rho%4 of lane 32*((rho/4)%2) + column.
+                const int twy = al >> 5, trb = (al >> 4) & 1, a16 = al & 15;
+                const int tcr = (bl >> 4) - 2 * hf, b16 = bl & 15;
+                const int t4 = trb * 2 + tcr;
+                uint32_t n4c[4], n4t[4];
+#pragma unroll
+                for (int pi = 0; pi < 2; pi++)
+#pragma unroll
+                    for (int pj = 0; pj < 2; pj++) {
+                        const int rho = 2 * a16 + pi;
+                        const int reg = 4 * (rho >> 3) + (rho & 3);
+                        const int ln = 32 * ((rho >> 2) & 1) + 2 * b16 + pj;
+                        const int off = ((t4 * 16) + reg) * 64 + ln;
+                        n4c[pi * 2 + pj] = smem[(0 * 4 + twy) * 4 * 16 * 64 + off];
+                        n4t[pi * 2 + pj] = smem[(1 * 4 + twy) * 4 * 16 * 64 + off];
+                    }
+                if constexpr (MODE == MODE_PAIRTABLE) {
+                    uint4 *o = p.pairs + ((size_t)A * p.n + B) * 2;
+                    o[0] = make_uint4(n4c[0], n4c[1], n4c[2], n4c[3]);
+                    o[1] = make_uint4(n4t[0], n4t[1], n4t[2], n4t[3]);
+                } else {
+                    const Singles sa = p.singles[A], sb = p.singles[B];
+                    uint32_t cas[9], ctl[9];
+                    pair9(n4c[0], n4c[1], n4c[2], n4c[3], sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
+                    pair9(n4t[0], n4t[1], n4t[2], n4t[3], sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);
+                    if constexpr (scores_sets(MODE)) {
+                        bad[m] = badness_f32(p.sc, cas, ctl);
+                    } else if constexpr (MODE == MODE_DUMP) {
+                        uint32_t *o = p.dump + rank2(A, B) * 18;
+                        for (int c = 0; c < 9; c++) {
+                            o[c] = cas[c];
+                            o[9 + c] = ctl[c];
+                        }
+                    } else if constexpr (MODE == MODE_CHECKSUM) {
+                        hsum += table_hash(rank2(A, B), cas, ctl, 9);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if constexpr (MODE == MODE_SEARCH) publish_candidates<16, 8>(p.co, p.sc.margin, bad, packed, s_red);
+        if constexpr (MODE == MODE_TOPK) publish_topk<16, 8>(p.co, p.sc.margin, bad, packed);
+        if constexpr (MODE == MODE_CHECKSUM) {
+            if (hsum) atomicAdd(p.checksum, hsum);
+        }
+    }
+}
+
+}  // namespace episdet

@@ -15,7 +15,12 @@ namespace episdet {
 constexpr int KW = 32;          // words per staged chunk (1024 samples)
 constexpr int LDSW = KW + 4;    // LDS row stride in words: conflict-free b128 reads
 
-enum { MODE_SEARCH = 0, MODE_DUMP = 1, MODE_CHECKSUM = 2, MODE_COUNTS = 3, MODE_PAIRTABLE = 4 };
+enum { MODE_SEARCH = 0, MODE_DUMP = 1, MODE_CHECKSUM = 2, MODE_COUNTS = 3, MODE_PAIRTABLE = 4, MODE_TOPK = 5 };
+
+// Modes that score sets: MODE_SEARCH keeps the best set's neighbourhood,
+// MODE_TOPK the k best sets' (a separate instantiation, so the single-best
+// kernels carry none of the top-k registers).
+constexpr bool scores_sets(int mode) { return mode == MODE_SEARCH || mode == MODE_TOPK; }
 
 // ------------------------------------------------------------ data prep --
 
@@ -70,7 +75,79 @@ struct CandOut {
     float *cand_s32;             // their FP32 score
     unsigned *cand_count;        // number appended (may exceed cand_cap: overflow)
     unsigned cand_cap;
+    unsigned topk;               // > 1: keep every set that can be among the topk best
 };
+
+// Top-k publication.  The k-th smallest FP32 badness among a block's sets
+// of one work unit is the badness of k distinct sets, so it bounds the
+// global k-th best from above.  It joins a global bound (atomicMin on
+// *best_key), and every finite set within `margin` of the bound is appended.
+// The bound only decreases, so each of the k best sets (by exact score) is
+// appended when it is evaluated.  The k-th smallest key is found by an
+// MSB-first radix select, one block-wide count per bit; it runs only when at
+// least k of the block's values lie below the current bound.  Requires
+// k <= M * blockDim.x (every kernel here has M * blockDim.x >= 1024).
+template <int M, int NW>
+__device__ void publish_topk(const CandOut &co, float margin, const float (&bad)[M],
+                             const unsigned long long (&packed)[M])
+{
+    __shared__ unsigned s_cnt[2][NW];
+    __shared__ unsigned s_g;
+    unsigned key[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) key[i] = f2key(bad[i]);     // +inf keys equal the initial bound: never counted
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    __syncthreads();                          // s_g / s_cnt of the previous call are consumed
+    if (tid == 0) s_g = __hip_atomic_load(co.best_key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    unsigned g = s_g;
+    unsigned mine = 0;
+#pragma unroll
+    for (int i = 0; i < M; i++) mine += key[i] < g;
+    mine = wave_sum_u(mine);
+    if (lane == 0) s_cnt[0][w] = mine;
+    __syncthreads();
+    unsigned c = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++) c += s_cnt[0][i];
+    if (c >= co.topk) {
+        unsigned prefix = 0, kk = co.topk;
+        for (int b = 31; b >= 0; b--) {
+            const unsigned hi = b == 31 ? 0u : (~0u << (b + 1));
+            unsigned z = 0;
+#pragma unroll
+            for (int i = 0; i < M; i++) z += ((key[i] ^ prefix) & hi) == 0 && ((key[i] >> b) & 1u) == 0;
+            z = wave_sum_u(z);
+            const int buf = b & 1;            // bit 31 uses buffer 1 (buffer 0 still holds c), then alternate
+            if (lane == 0) s_cnt[buf][w] = z;
+            __syncthreads();
+            unsigned cnt = 0;
+#pragma unroll
+            for (int i = 0; i < NW; i++) cnt += s_cnt[buf][i];
+            if (kk > cnt) {
+                kk -= cnt;
+                prefix |= 1u << b;
+            }
+        }
+        if (tid == 0) {
+            const unsigned old = atomicMin(co.best_key, prefix);
+            s_g = old < prefix ? old : prefix;
+        }
+        __syncthreads();
+        g = s_g;
+    }
+    const float thr = g >= 0xFF800000u ? INFINITY : key2f(g) + margin;
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+        if (bad[i] <= thr && bad[i] < INFINITY) {
+            unsigned slot = atomicAdd(co.cand_count, 1u);
+            if (slot < co.cand_cap) {
+                co.cand[slot] = packed[i];
+                co.cand_s32[slot] = bad[i];
+            }
+        }
+    }
+}
 
 // Block-wide candidate publication for a work unit.  bad[] holds the
 // thread's FP32 badness values (+inf for invalid sets).  The block minimum
@@ -258,7 +335,7 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
                           (uint32_t)p.N0, cas);
                     pair9(acc1[r][s][0], acc1[r][s][1], acc1[r][s][2], acc1[r][s][3], sa.z, sa.w, sb.z, sb.w,
                           (uint32_t)p.N1, ctl);
-                    if constexpr (MODE == MODE_SEARCH) {
+                    if constexpr (scores_sets(MODE)) {
                         bad[m] = badness_f32(p.sc, cas, ctl);
                     } else if constexpr (MODE == MODE_DUMP) {
                         uint32_t *o = p.dump + rank2(A, B) * 18;
@@ -273,6 +350,7 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
             }
         }
         if constexpr (MODE == MODE_SEARCH) publish_candidates<R * R>(p.co, p.sc.margin, bad, packed, s_red);
+        if constexpr (MODE == MODE_TOPK) publish_topk<R * R, 4>(p.co, p.sc.margin, bad, packed);
         if constexpr (MODE == MODE_CHECKSUM) {
             if (hsum) atomicAdd(p.checksum, hsum);
         }
@@ -491,7 +569,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
                     if (!(vy && Y < Z && Z < p.n)) continue;
                     uint32_t cas[27], ctl[27];
                     triplet_tables(p, X, Y, Z, sx, sy, acc0[a][b], acc1[a][b], cas, ctl);
-                    if constexpr (MODE == MODE_SEARCH) {
+                    if constexpr (scores_sets(MODE)) {
                         bad[m] = badness_f32(p.sc, cas, ctl);
                     } else if constexpr (MODE == MODE_DUMP) {
                         uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
@@ -505,6 +583,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
                 }
             }
             if constexpr (MODE == MODE_SEARCH) publish_candidates<A * B>(p.co, p.sc.margin, bad, packed, s_red);
+            if constexpr (MODE == MODE_TOPK) publish_topk<A * B, 4>(p.co, p.sc.margin, bad, packed);
             if constexpr (MODE == MODE_CHECKSUM) {
                 if (hsum) atomicAdd(p.checksum, hsum);
             }

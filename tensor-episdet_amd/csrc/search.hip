@@ -20,6 +20,7 @@
 
 #include "episdet.h"
 #include "kernels_mfma.hpp"
+#include "kernels_mfma_pairs.hpp"
 
 using namespace episdet;
 
@@ -84,6 +85,8 @@ struct episdet_dataset {
     int cu = 0;
     void *jobargs = nullptr;                  // device copy of one kernel argument block
     unsigned long long *jobstart = nullptr;   // {0, units} for a batch of one
+    std::vector<unsigned long long> topk_packed;   // last top-k result, best first
+    std::vector<double> topk_scores;
 };
 
 namespace {
@@ -290,7 +293,23 @@ int launch_pairs_batch(int R, int cu, const BatchPair &bp, hipStream_t s)
 }
 
 template <int MODE>
-int dispatch_pairs(episdet_dataset *ds, int tile, PairArgs &a, hipStream_t s)
+int launch_pairs_mfma(int cu, const BatchPair &bp, hipStream_t s)
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_pairs_mfma<MODE>, 512, 0) != hipSuccess ||
+        per < 1)
+        per = 1;
+    hipLaunchKernelGGL((k_pairs_mfma<MODE>), dim3(per * cu), dim3(512), 0, s, bp);
+    HIPCHK(hipGetLastError());
+    return EPISDET_OK;
+}
+
+// PairArgs for the FP4-MFMA pair kernel (128 x 64 units, B-tile-major) or
+// the VALU pair kernel (square 16R tiles, triangular).
+void fill_pairs(episdet_dataset *ds, PairArgs &a, bool mfma, int tile);
+
+template <int MODE>
+int dispatch_pairs(episdet_dataset *ds, bool mfma, int tile, PairArgs &a, hipStream_t s)
 {
     CHK(upload_single(ds, a, a.nunits));
     BatchPair bp;
@@ -299,6 +318,7 @@ int dispatch_pairs(episdet_dataset *ds, int tile, PairArgs &a, hipStream_t s)
     bp.njobs = 1;
     bp.total = a.nunits;
     bp.queue = a.queue;
+    if (mfma) return launch_pairs_mfma<MODE>(ds->cu, bp, s);
     return launch_pairs_batch<MODE>(PAIR_TILES[tile], ds->cu, bp, s);
 }
 
@@ -368,6 +388,16 @@ void fill_pair_common(episdet_dataset *ds, PairArgs &a, int T)
     a.co.cand_cap = ds->cand_cap;
 }
 
+void fill_pairs(episdet_dataset *ds, PairArgs &a, bool mfma, int tile)
+{
+    if (mfma) {
+        fill_pair_common(ds, a, P2_TB);
+        a.nunits = p2_units(a.ntiles);
+    } else {
+        fill_pair_common(ds, a, 16 * PAIR_TILES[tile]);
+    }
+}
+
 int init_ctl(episdet_dataset *ds)
 {
     hipLaunchKernelGGL(k_init_search, dim3(1), dim3(64), 0, ds->stream, &ds->ctl->queue, &ds->ctl->best32,
@@ -386,9 +416,10 @@ int ensure_pairs(episdet_dataset *ds)
     }
     CHK(init_ctl(ds));
     PairArgs a;
-    fill_pair_common(ds, a, 16 * PAIR_TILES[0]);
+    const bool mfma = mfma_ok(ds);
+    fill_pairs(ds, a, mfma, 0);
     a.pairs = ds->pairs;
-    CHK(dispatch_pairs<MODE_PAIRTABLE>(ds, 0, a, ds->stream));
+    CHK(dispatch_pairs<MODE_PAIRTABLE>(ds, mfma, 0, a, ds->stream));
     HIPCHK(hipStreamSynchronize(ds->stream));
     return EPISDET_OK;
 }
@@ -495,20 +526,60 @@ int collect_best(episdet_dataset *ds, int order, int score, int count, episdet_r
     return EPISDET_OK;
 }
 
-// Fused search (VALU): one persistent launch, then FP64 re-scoring.
+// The k best candidates by FP64 score (better() order), best first.
+int collect_topk(episdet_dataset *ds, int order, int score, int count, int k, episdet_result *res)
+{
+    std::vector<unsigned long long> hp(count);
+    std::vector<double> h64(count);
+    HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
+    std::vector<int> ord(count);
+    for (int i = 0; i < count; i++) ord[i] = i;
+    const int m = std::min(k, count);
+    std::partial_sort(ord.begin(), ord.begin() + m, ord.end(),
+                      [&](int a, int b) { return better(score, h64[a], hp[a], h64[b], hp[b]); });
+    ds->topk_packed.resize(m);
+    ds->topk_scores.resize(m);
+    for (int i = 0; i < m; i++) {
+        ds->topk_packed[i] = hp[ord[i]];
+        ds->topk_scores[i] = h64[ord[i]];
+    }
+    (void)order;
+    (void)res;
+    return EPISDET_OK;
+}
+
+// AUTO resolves to the FP4-MFMA kernels wherever they are exact (fewer than
+// 2^24 samples per class), else to the VALU kernels.
+int resolve_path(const episdet_dataset *ds, const episdet_options *o)
+{
+    if (o->path != EPISDET_PATH_AUTO) return o->path;
+    return mfma_ok(ds) ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
+}
+
+// Fused search: one persistent launch, then FP64 re-scoring of the
+// candidates.  With o->topk > 1 the candidates cover the topk best sets.
 int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
 {
     const int order = o->order, score = o->score;
-    const bool mfma = o->path == EPISDET_PATH_FUSED_MFMA;
-    if (mfma && (order != 3 || !mfma_ok(ds))) return EPISDET_ERR_INVALID;
+    const int path = resolve_path(ds, o);
+    const bool mfma = path == EPISDET_PATH_FUSED_MFMA;
+    if (mfma && !mfma_ok(ds)) return EPISDET_ERR_INVALID;
     const int tile = o->tile;
-    if (mfma && (tile < 0 || tile >= MFMA_TILES)) return EPISDET_ERR_INVALID;
+    if (mfma && (tile < 0 || tile >= (order == 3 ? MFMA_TILES : 1))) return EPISDET_ERR_INVALID;
     if (!mfma && order == 3 && (tile < 0 || tile >= (int)(sizeof TRI_TILES / sizeof TRI_TILES[0])))
         return EPISDET_ERR_INVALID;
-    if (order == 2 && (tile < 0 || tile >= (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0]))) return EPISDET_ERR_INVALID;
+    if (!mfma && order == 2 && (tile < 0 || tile >= (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0])))
+        return EPISDET_ERR_INVALID;
+    const int topk = o->topk > 1 ? o->topk : 0;
+    if (topk > EPISDET_TOPK_MAX) return EPISDET_ERR_INVALID;
     CHK(ensure_tables(ds, score));
     if (order == 3) CHK(ensure_pairs(ds));
-    CHK(ensure_cand(ds, o->candidate_cap > 0 ? (unsigned)o->candidate_cap : DEFAULT_CAP));
+    unsigned cap = o->candidate_cap > 0 ? (unsigned)o->candidate_cap : DEFAULT_CAP;
+    // top-k: before the bound settles, each resident block can append about
+    // 2k sets of its first units
+    if (topk) cap = std::max<unsigned>(cap, (unsigned)std::min<unsigned long long>(1ull << 26, 32ull * topk * ds->cu));
+    CHK(ensure_cand(ds, cap));
     const ScoreCtx sc = make_ctx(ds, score, order);
 
     unsigned long long *d_gstart = nullptr;
@@ -539,36 +610,65 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
         ta.ngroups = (int)groups.size();
         ta.nunits = gstart.back();
     } else {
-        fill_pair_common(ds, pa, 16 * PAIR_TILES[tile]);
+        fill_pairs(ds, pa, mfma, tile);
         pa.sc = sc;
     }
 
-    CHK(init_ctl(ds));
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, ds->stream));
-    int rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream)
-                                : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
-                        : dispatch_pairs<MODE_SEARCH>(ds, tile, pa, ds->stream);
-    HIPCHK(hipEventRecord(e1, ds->stream));
-    HIPCHK(hipEventSynchronize(e1));
+    // One search launch; with top-k, a candidate overflow grows the buffer
+    // and launches again (at most twice).
+    Ctl hc;
+    int rc = EPISDET_OK;
+    double kms_total = 0;
+    for (int attempt = 0;; attempt++) {
+        ta.co.cand = pa.co.cand = ds->cand;
+        ta.co.cand_s32 = pa.co.cand_s32 = ds->cand_s32;
+        ta.co.cand_cap = pa.co.cand_cap = ds->cand_cap;
+        ta.co.topk = pa.co.topk = (unsigned)topk;
+        rc = init_ctl(ds);
+        if (rc != EPISDET_OK) break;
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            rc = EPISDET_ERR_HIP;
+            break;
+        }
+        (void)hipEventRecord(e0, ds->stream);
+        if (topk)
+            rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_TOPK>(ds, tile, ta, ds->stream)
+                                    : dispatch_tri<MODE_TOPK>(ds, tile, ta, 0, ds->stream))
+                            : dispatch_pairs<MODE_TOPK>(ds, mfma, tile, pa, ds->stream);
+        else
+            rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream)
+                                    : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
+                            : dispatch_pairs<MODE_SEARCH>(ds, mfma, tile, pa, ds->stream);
+        (void)hipEventRecord(e1, ds->stream);
+        const bool synced = hipEventSynchronize(e1) == hipSuccess;
+        float kms = 0;
+        if (synced) (void)hipEventElapsedTime(&kms, e0, e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc == EPISDET_OK && !synced) rc = EPISDET_ERR_HIP;
+        if (rc != EPISDET_OK) break;
+        kms_total += kms;
+        if (hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = EPISDET_ERR_HIP;
+            break;
+        }
+        if (!topk || hc.cand_count <= ds->cand_cap || attempt == 2) break;
+        const unsigned long long grow = std::min<unsigned long long>(1ull << 28, 4ull * hc.cand_count);
+        if (grow <= ds->cand_cap) break;
+        rc = ensure_cand(ds, (unsigned)grow);
+        if (rc != EPISDET_OK) break;
+    }
     if (d_gstart) (void)hipFree(d_gstart);
     if (d_groups) (void)hipFree(d_groups);
     if (rc != EPISDET_OK) return rc;
-    float kms = 0;
-    HIPCHK(hipEventElapsedTime(&kms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    res->kernel_ms = kms;
+    res->kernel_ms = kms_total;
     res->margin = sc.margin;
     res->tile = tile;
-
-    Ctl hc;
-    HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
     res->candidates = hc.cand_count;
     if (hc.cand_count == 0) return EPISDET_ERR_INTERNAL;
     if (hc.cand_count > ds->cand_cap) {
+        if (topk) return EPISDET_ERR_CANDIDATES;
         // Too many sets within the FP32 margin (heavy ties, degenerate data):
         // answer exactly with the FP64 round loop instead (3-way).
         if (order != 3) return EPISDET_ERR_CANDIDATES;
@@ -579,7 +679,7 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
         CHK(run_plain(ds, &po, res));
         res->kernel_ms += k;
         res->fallback = 1;
-        res->path = o->path;
+        res->path = path;
         return EPISDET_OK;
     }
     const int count = (int)hc.cand_count;
@@ -587,7 +687,8 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     CHK(rescore(ds, order, score, count, &rms));
     res->rescore_ms = rms;
     CHK(collect_best(ds, order, score, count, res));
-    res->path = mfma ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
+    if (topk) CHK(collect_topk(ds, order, score, count, topk, res));
+    res->path = path;
     return EPISDET_OK;
 }
 
@@ -879,6 +980,7 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
     if (opt->order != 2 && opt->order != 3) return EPISDET_ERR_INVALID;
     if (opt->score != EPISDET_SCORE_K2 && opt->score != EPISDET_SCORE_MI) return EPISDET_ERR_INVALID;
     if (ds->n < opt->order) return EPISDET_ERR_INVALID;
+    if (opt->topk > 1 && (opt->path == EPISDET_PATH_PLAIN || opt->topk > EPISDET_TOPK_MAX)) return EPISDET_ERR_INVALID;
     std::memset(res, 0, sizeof *res);
     const double t0 = now_s();
     if (opt->rebuild) {
@@ -904,6 +1006,31 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
     if (opt->path == EPISDET_PATH_PLAIN && opt->order == 3 && res->rounds_run < res->rounds && res->rounds > 0)
         res->sets = (long long)((double)res->sets * (double)res->rounds_run / (double)res->rounds);
     return rc;
+}
+
+int episdet_run_topk(episdet_dataset *ds, const episdet_options *opt, int k, unsigned long long *packed,
+                     double *scores, int *count, episdet_result *res)
+{
+    if (!ds || !opt || !packed || !scores || !count || !res || k < 1 || k > EPISDET_TOPK_MAX)
+        return EPISDET_ERR_INVALID;
+    episdet_options o = *opt;
+    o.topk = k;
+    ds->topk_packed.clear();
+    ds->topk_scores.clear();
+    CHK(episdet_run(ds, &o, res));
+    if (k == 1) {
+        packed[0] = res->packed;
+        scores[0] = res->best_score;
+        *count = 1;
+        return EPISDET_OK;
+    }
+    const int m = (int)ds->topk_packed.size();
+    for (int i = 0; i < m; i++) {
+        packed[i] = ds->topk_packed[i];
+        scores[i] = ds->topk_scores[i];
+    }
+    *count = m;
+    return EPISDET_OK;
 }
 
 int episdet_search(int order, int score, const unsigned *cases, const unsigned *controls, int numSNPs, int numCases,
@@ -947,7 +1074,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         const episdet_options &o = opts[j];
         std::memset(&results[j], 0, sizeof(episdet_result));
         if (!ds || (o.order != 2 && o.order != 3) || (o.score != EPISDET_SCORE_K2 && o.score != EPISDET_SCORE_MI) ||
-            ds->n < o.order)
+            ds->n < o.order || o.topk > 1)
             return EPISDET_ERR_INVALID;
         const double tp = now_s();
         if (o.rebuild) {
@@ -962,11 +1089,11 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         CHK(ensure_cand(ds, o.candidate_cap > 0 ? (unsigned)o.candidate_cap : DEFAULT_CAP));
         results[j].prep_ms = (now_s() - tp) * 1e3;
         // 3-way jobs go to the batched FP4 MFMA kernel, 2-way jobs to the
-        // batched VALU kernel; anything else runs on its own afterwards.
+        // batched FP4-MFMA pair kernel; anything else runs on its own afterwards.
         // A dataset's control block and candidate buffer hold one search at a
         // time, so a second job on the same dataset runs after the graph.
         const bool dup = std::find(seen.begin(), seen.end(), ds) != seen.end();
-        if (dup || o.path == EPISDET_PATH_PLAIN || (o.order == 3 && !mfma_ok(ds))) {
+        if (dup || o.path == EPISDET_PATH_PLAIN || !mfma_ok(ds)) {
             solo.push_back(j);
             continue;
         }
@@ -1000,7 +1127,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
     std::vector<unsigned long long> pstart(1, 0ull);
     for (size_t k = 0; k < pairs.size(); k++) {
         episdet_dataset *ds = dss[pairs[k]];
-        fill_pair_common(ds, pa[k], 16 * PAIR_TILES[0]);
+        fill_pairs(ds, pa[k], true, 0);
         pa[k].sc = make_ctx(ds, opts[pairs[k]].score, 2);
         pstart.push_back(pstart.back() + pa[k].nunits);
     }
@@ -1057,16 +1184,27 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
                 hipSuccess ||
             per3 < 1)
             per3 = 1;
-        per2 = launch_blocks((const void *)k_pairs_valu<4, MODE_SEARCH>, cu) / cu;
-        hipStream_t st;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, (const void *)k_pairs_mfma<MODE_SEARCH>, 512, 0) !=
+                hipSuccess ||
+            per2 < 1)
+            per2 = 1;
+        hipStream_t st, st2;
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        hipEvent_t fork, join;
+        HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
         hipGraph_t graph;
         hipGraphExec_t exec;
-        // one graph: reset, 3-way batch, 2-way batch, FP64 re-scoring
+        // one graph: reset, then the 3-way and the 2-way batch as parallel
+        // branches (the second fills CUs the first's tail leaves idle), then
+        // FP64 re-scoring
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const int nb = (int)batched.size();
         hipLaunchKernelGGL(k_init_many, dim3((std::max(nb, 2) + 63) / 64), dim3(64), 0, st, d_qp, d_bp, d_cp, nb, d_bq,
                            2);
+        HIPCHK(hipEventRecord(fork, st));
+        HIPCHK(hipStreamWaitEvent(st2, fork, 0));
         if (!tri.empty()) {
             BatchTri bt;
             bt.jobs = d_ta;
@@ -1083,8 +1221,10 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             bpr.njobs = (int)pairs.size();
             bpr.total = pstart.back();
             bpr.queue = d_bq + 1;
-            hipLaunchKernelGGL((k_pairs_valu<4, MODE_SEARCH>), dim3(per2 * cu), dim3(256), 0, st, bpr);
+            hipLaunchKernelGGL((k_pairs_mfma<MODE_SEARCH>), dim3(per2 * cu), dim3(512), 0, st2, bpr);
         }
+        HIPCHK(hipEventRecord(join, st2));
+        HIPCHK(hipStreamWaitEvent(st, join, 0));
         hipLaunchKernelGGL(k_rescore_batch, dim3(8 * cu), dim3(64), 0, st, d_rj, nb);
         HIPCHK(hipStreamEndCapture(st, &graph));
         HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1102,6 +1242,9 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         (void)hipEventDestroy(e1);
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
+        (void)hipEventDestroy(fork);
+        (void)hipEventDestroy(join);
+        (void)hipStreamDestroy(st2);
         (void)hipStreamDestroy(st);
     }
     for (size_t k = 0; k < batched.size(); k++) {
@@ -1214,7 +1357,7 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
     if (path != EPISDET_PATH_FUSED_VALU && path != EPISDET_PATH_AUTO && path != EPISDET_PATH_FUSED_MFMA)
         return EPISDET_ERR_INVALID;
     const bool mfma = path == EPISDET_PATH_FUSED_MFMA;
-    if (mfma && (order != 3 || !mfma_ok(ds))) return EPISDET_ERR_INVALID;
+    if (mfma && !mfma_ok(ds)) return EPISDET_ERR_INVALID;
     const int nc = order == 3 ? 27 : 9;
     const unsigned long long total = episdet_n_choose_k((unsigned)ds->n, (unsigned)order);
     uint32_t *d_dump = nullptr;
@@ -1256,9 +1399,10 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
                      : dispatch_tri<MODE_CHECKSUM>(ds, 0, ta, 0, ds->stream);
     } else {
         PairArgs pa;
-        fill_pair_common(ds, pa, 16 * PAIR_TILES[0]);
+        fill_pairs(ds, pa, mfma, 0);
         pa.dump = d_dump;
-        rc = out ? dispatch_pairs<MODE_DUMP>(ds, 0, pa, ds->stream) : dispatch_pairs<MODE_CHECKSUM>(ds, 0, pa, ds->stream);
+        rc = out ? dispatch_pairs<MODE_DUMP>(ds, mfma, 0, pa, ds->stream)
+                 : dispatch_pairs<MODE_CHECKSUM>(ds, mfma, 0, pa, ds->stream);
     }
     HIPCHK(hipStreamSynchronize(ds->stream));
     if (d_gstart) (void)hipFree(d_gstart);

@@ -24,6 +24,7 @@ BIN_DIR = os.path.join(_HERE, "bin")
 
 K2, MI = 0, 1
 PATH_AUTO, PATH_FUSED_VALU, PATH_FUSED_MFMA, PATH_PLAIN = 0, 1, 2, 3
+TOPK_MAX = 1024   # EPISDET_TOPK_MAX
 MAF_UNIFORM, MAF_BETA = 0, 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "no gfx950 device", 3: "HIP runtime error",
@@ -50,7 +51,7 @@ def _load(name):
 class Options(C.Structure):
     _fields_ = [("order", C.c_int), ("score", C.c_int), ("path", C.c_int), ("tile", C.c_int),
                 ("plain_rounds", C.c_int), ("candidate_cap", C.c_int), ("rebuild", C.c_int),
-                ("reserved", C.c_int * 7)]
+                ("topk", C.c_int), ("reserved", C.c_int * 6)]
 
 
 class Result(C.Structure):
@@ -88,7 +89,7 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 # Every symbol include/episdet.h and include/episdet_data.h declare.
 SEARCH_SYMBOLS = ("episdet_search", "episdet_dataset_create", "episdet_dataset_destroy", "episdet_run",
-                  "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
+                  "episdet_run_topk", "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
                   "episdet_checksum", "episdet_n_choose_k", "episdet_reference_rounds", "episdet_status_string",
                   "episdet_device_info")
 DATA_SYMBOLS = ("episdet_gen_defaults", "episdet_generate", "episdet_padded_words", "episdet_write_dataset",
@@ -133,6 +134,9 @@ def search_lib():
         lib.episdet_dataset_destroy.restype = None
         lib.episdet_run.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result)]
         lib.episdet_run.restype = C.c_int
+        lib.episdet_run_topk.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, C.POINTER(C.c_ulonglong),
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(Result)]
+        lib.episdet_run_topk.restype = C.c_int
         lib.episdet_run_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Options), C.c_int, C.POINTER(Result),
                                           C.POINTER(C.c_double)]
         lib.episdet_run_batch.restype = C.c_int
@@ -348,6 +352,20 @@ class Dataset:
         if rc:
             raise EpisdetError(rc, "episdet_run")
         return r.as_dict()
+
+    def topk(self, order, k, score=K2, path=PATH_AUTO, tile=0, rebuild=False):
+        """The k best sets by FP64 score (best first): (packed list, scores
+        list, result dict)."""
+        o = Options(order=order, score=score, path=path, tile=tile, rebuild=1 if rebuild else 0)
+        packed = (C.c_ulonglong * k)()
+        scores = (C.c_double * k)()
+        cnt = C.c_int()
+        r = Result()
+        rc = self._lib.episdet_run_topk(self._h, C.byref(o), k, packed, scores, C.byref(cnt), C.byref(r))
+        if rc:
+            raise EpisdetError(rc, "episdet_run_topk")
+        m = cnt.value
+        return [int(x) for x in packed[:m]], [float(x) for x in scores[:m]], r.as_dict()
 
     def tables(self, order, sets) -> np.ndarray:
         sets = np.ascontiguousarray(np.asarray(sets, dtype=np.int32).reshape(-1, order))

@@ -80,7 +80,8 @@ CASES = [
 def test_fused_tables_bit_exact(n, Nc, Nt, wc, wt, seed):
     syn = episdet.generate(n, Nc, Nt, seed=seed, cases_words=wc, controls_words=wt)
     with episdet.Dataset.from_synthetic(syn) as ds:
-        for order, path in ((2, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_MFMA)):
+        for order, path in ((2, episdet.PATH_FUSED_VALU), (2, episdet.PATH_FUSED_MFMA), (3, episdet.PATH_FUSED_VALU),
+                            (3, episdet.PATH_FUSED_MFMA)):
             say(f"dump order={order} path={path} n={n} Nc={Nc} Nt={Nt}")
             dump = ds.dump_tables(order, path=path)
             ref, cs = O.all_tables(syn.cases, syn.controls, Nc, Nt, order)
@@ -94,7 +95,8 @@ def test_fused_best_matches_oracle(n, Nc, Nt, wc, wt, seed):
     syn = episdet.generate(n, Nc, Nt, seed=seed, cases_words=wc, controls_words=wt,
                            planted_sets=1 if n >= 60 else 0, planted_order=3)
     with episdet.Dataset.from_synthetic(syn) as ds:
-        for order, path in ((2, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_VALU), (3, episdet.PATH_FUSED_MFMA)):
+        for order, path in ((2, episdet.PATH_FUSED_VALU), (2, episdet.PATH_FUSED_MFMA), (3, episdet.PATH_FUSED_VALU),
+                            (3, episdet.PATH_FUSED_MFMA)):
             for score in (episdet.K2, episdet.MI):
                 r = check_best(ds, syn.cases, syn.controls, Nc, Nt, order, score, path=path)
                 if score == episdet.K2 and r["fallback"] == 0:
@@ -194,14 +196,15 @@ def test_config1_all_pair_tables():
     bit-exact and the best pair identical."""
     syn = episdet.generate(512, 1024, 1024, seed=1)
     with episdet.Dataset.from_synthetic(syn) as ds:
-        say("config1 dump")
-        dump = ds.dump_tables(2)
         ref, cs = O.all_tables(syn.cases, syn.controls, 1024, 1024, 2)
-        assert dump.shape[0] == 130_816
-        assert (dump == ref).all()
-        assert ds.checksum(2) == cs
-        for score in (episdet.K2, episdet.MI):
-            check_best(ds, syn.cases, syn.controls, 1024, 1024, 2, score)
+        for path in (episdet.PATH_FUSED_VALU, episdet.PATH_FUSED_MFMA):
+            say(f"config1 dump path={path}")
+            dump = ds.dump_tables(2, path=path)
+            assert dump.shape[0] == 130_816
+            assert (dump == ref).all()
+            assert ds.checksum(2, path=path) == cs
+            for score in (episdet.K2, episdet.MI):
+                check_best(ds, syn.cases, syn.controls, 1024, 1024, 2, score, path=path)
 
 
 def test_full_sample_axis_planted_triplet():
@@ -229,8 +232,11 @@ def test_mi_pairs_uneven_classes_planted():
     syn = episdet.generate(1024, 26214, 39322, seed=3, maf_dist=episdet.MAF_BETA, maf=(0.01, 0.5),
                            planted_sets=1, planted_order=2)
     with episdet.Dataset.from_synthetic(syn) as ds:
-        r = check_best(ds, syn.cases, syn.controls, 26214, 39322, 2, episdet.MI)
-        assert tuple(r["idx"][:2]) == syn.planted[0]
+        for path in (episdet.PATH_FUSED_VALU, episdet.PATH_FUSED_MFMA):
+            r = check_best(ds, syn.cases, syn.controls, 26214, 39322, 2, episdet.MI, path=path)
+            assert tuple(r["idx"][:2]) == syn.planted[0]
+        say("uneven classes: VALU and MFMA pair checksums")
+        assert ds.checksum(2, path=episdet.PATH_FUSED_MFMA) == ds.checksum(2, path=episdet.PATH_FUSED_VALU)
 
 
 # ------------------------------------------------------------- batches --
@@ -268,3 +274,47 @@ def test_batch_of_uneven_jobs_matches_oracle():
     assert res3[1]["packed"] == o["packed"] and res3[1]["best_score"] == o["score"]
     for d in dss:
         d.close()
+
+
+# ------------------------------------------------------------- top-k ----
+
+TOPK_CASES = [
+    # order, path, tile, n, Nc, Nt, seed
+    (2, episdet.PATH_FUSED_VALU, 0, 150, 900, 1100, 31),
+    (2, episdet.PATH_FUSED_MFMA, 0, 200, 1100, 700, 36),
+    (3, episdet.PATH_FUSED_VALU, 0, 45, 1500, 1300, 32),
+    (3, episdet.PATH_FUSED_MFMA, 0, 100, 2000, 1800, 33),
+    (3, episdet.PATH_FUSED_MFMA, 1, 60, 700, 2100, 34),
+]
+
+
+@pytest.mark.parametrize("order,path,tile,n,Nc,Nt,seed", TOPK_CASES)
+def test_topk_matches_oracle_ranking(order, path, tile, n, Nc, Nt, seed):
+    """The k best sets, best first, with FP64 scores bit-identical to the
+    oracle's full ranking (ties by packed index), for k from 1 to 1000."""
+    syn = episdet.generate(n, Nc, Nt, seed=seed, planted_sets=1, planted_order=order)
+    cases, controls = syn.cases.copy(), syn.controls.copy()
+    cases[7], controls[7] = cases[3], controls[3]          # exact ties in the ranking
+    with episdet.Dataset(cases, controls, Nc, Nt) as ds:
+        for score in (episdet.K2, episdet.MI):
+            ref_p, ref_s = O.ranked(cases, controls, Nc, Nt, order, score)
+            for k in (1, 2, 10, 100, 1000):
+                say(f"topk order={order} path={path} tile={tile} n={n} score={score} k={k}")
+                p, s, r = ds.topk(order, k, score=score, path=path, tile=tile)
+                assert len(p) == k
+                assert p == [int(x) for x in ref_p[:k]]
+                assert s == [float(x) for x in ref_s[:k]]
+                assert r["packed"] == p[0] and r["best_score"] == s[0]
+
+
+def test_topk_more_than_all_sets_and_errors():
+    syn = episdet.generate(6, 400, 500, seed=35)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for order in (2, 3):
+            ref_p, ref_s = O.ranked(syn.cases, syn.controls, 400, 500, order, episdet.K2)
+            p, s, _ = ds.topk(order, 50)
+            assert p == [int(x) for x in ref_p] and s == [float(x) for x in ref_s]
+        with pytest.raises(episdet.EpisdetError):
+            ds.topk(3, 10, path=episdet.PATH_PLAIN)
+        with pytest.raises(episdet.EpisdetError):
+            ds.topk(3, episdet.TOPK_MAX + 1)

@@ -207,3 +207,32 @@ def test_oracle_recovers_planted_pair():
     for score in (O.K2, O.MI):
         r = O.search(syn.cases, syn.controls, 3000, 4500, 2, score)
         assert tuple(r["idx"]) == syn.planted[0]
+
+
+@pytest.mark.parametrize("order,score", [(2, 0), (2, 1), (3, 0), (3, 1)])
+def test_ranked_agrees_with_search_and_direct_counting(order, score):
+    """oracle.ranked (used to check top-k) puts orc_search's winner first,
+    is monotone, breaks exact ties by packed index, and scores each set as a
+    per-sample genotype count does."""
+    rng = np.random.default_rng(40 + order + 2 * score)
+    n, Nc, Nt = (14, 300, 260) if order == 3 else (30, 300, 260)
+    gc, cases = rand_planes(rng, n, Nc)
+    gt, controls = rand_planes(rng, n, Nt)
+    cases[5], controls[5] = cases[2], controls[2]          # a duplicated SNP makes exact ties
+    gc[5], gt[5] = gc[2], gt[2]
+    packed, sc = O.ranked(cases, controls, Nc, Nt, order, score)
+    assert len(packed) == math.comb(n, order)
+    best = O.search(cases, controls, Nc, Nt, order, score)
+    assert int(packed[0]) == best["packed"] and sc[0] == best["score"]
+    d = np.diff(sc)
+    assert (d >= 0).all() if score == 0 else (d <= 0).all()
+    tie = d == 0
+    assert tie.any() and (np.diff(packed.astype(np.int64))[tie] > 0).all()
+    T = O.lnfact_table(Nc + Nt + 1) if score == 0 else O.log2_table(Nc + Nt)
+    for i in list(range(5)) + list(rng.integers(0, len(packed), 20)):
+        p = int(packed[i])
+        s = [p & 0x1FFFFF, (p >> 21) & 0x1FFFFF, p >> 42] if order == 3 else [p & 0xFFFFFFFF, p >> 32]
+        a = O.table_direct(gc, Nc, s)
+        b = O.table_direct(gt, Nt, s)
+        ref = O.score_k2(a, b, T) if score == 0 else O.score_mi(a, b, T)
+        assert ref == sc[i]
