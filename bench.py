@@ -241,15 +241,22 @@ def main():
         def sets_in(lo, hi):
             return sum(math.comb(z, order - 1) for z in range(lo, hi))
 
+        def span(lo, want):
+            # smallest multiple of the thread count covering `want` sets
+            hi, acc = lo, 0
+            while hi < n and (acc < want or (hi - lo) % threads):
+                acc += math.comb(hi, order - 1)
+                hi += 1
+            return hi
+
         z0 = max(order, min(n // 4, 600))
         probe_t = []
-        ks = (threads, 3 * threads)
-        for k in ks:
+        ends = (span(z0, 1_000_000), span(z0, 3_000_000))
+        for hi in ends:
             t0 = time.perf_counter()
-            oracle.search(syn.cases, syn.controls, Nc, Nt, order, score, threads=threads,
-                          last=(z0, min(n, z0 + k)))
+            oracle.search(syn.cases, syn.controls, Nc, Nt, order, score, threads=threads, last=(z0, hi))
             probe_t.append(time.perf_counter() - t0)
-        s1, s2 = sets_in(z0, min(n, z0 + ks[0])), sets_in(z0, min(n, z0 + ks[1]))
+        s1, s2 = sets_in(z0, ends[0]), sets_in(z0, ends[1])
         per_set = max((probe_t[1] - probe_t[0]) / max(s2 - s1, 1), probe_t[0] / max(s1, 1) / 4, 1e-12)
         target = args.cpu_seconds / per_set
         zn, acc = 0, 0

@@ -13,9 +13,10 @@ enum { SCORE_K2 = 0, SCORE_MI = 1 };
 struct ScoreCtx {
     int score;          // SCORE_K2 / SCORE_MI
     int ncells;         // 9 or 27
-    const float *t32;   // K2: ln(m!) - c*m (shifted, see k2_f32); MI: log2(m/N)
-    float k2_total;     // K2: ncells * c
-    float invN;         // MI: 1/N
+    float k2_total;     // K2: ncells * c, c = ln N - 1
+    float k0;           // K2: ln(N)/2 - 1 + ln(2 pi)/2 (see k2_term)
+    float c;            // K2: c = ln N - 1
+    float invN;         // 1/N
     float hY;           // MI: H(Y) in bits
     float margin;       // candidate margin, 2 x the FP32 error bound
 };
@@ -86,34 +87,66 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
     s = t;
 }
 
+constexpr float LN2_F = 0.693147180559945309f;
+
+// K2 term T'(m) = ln m! - c*m, c = ln N - 1, computed without memory access
+// from Stirling's series for ln Gamma(x), x = m + 1, with the shift folded
+// in:
+//   T'(m) = (m + 1/2) ln(x/N) + K0 + 1/(12x) - 1/(360x^3) + 1/(1260x^5),
+//   K0 = ln(N)/2 - 1 + ln(2 pi)/2,
+// and exactly T'(0) = 0, T'(1) = -c.  The truncation error is below
+// 1/(1680x^7) (5e-6 at m = 2); ln(x/N) comes from v_log_f32.  Writing the
+// shift into ln(x/N) keeps every intermediate near |T'| <= N/e instead of
+// N ln N, so the FP32 error stays about an ulp of |T'|.  The host measures
+// the error of exactly this code against the FP64 table over every m a
+// search can reach (k_term_error) and widens the candidate margin by it.
+// No contraction, so every instantiation rounds identically.
+__device__ __forceinline__ float k2_term(uint32_t m, float invN, float k0, float c)
+{
+#pragma clang fp contract(off)
+    const float x = (float)(m + 1u);
+    const float l2 = __builtin_amdgcn_logf(x * invN);
+    const float ix = __builtin_amdgcn_rcpf(x);
+    const float ix2 = ix * ix;
+    const float corr = ix * (1.f / 12.f - ix2 * (1.f / 360.f - ix2 * (1.f / 1260.f)));
+    const float t = (((float)m + 0.5f) * LN2_F) * l2 + (k0 + corr);
+    return m > 1u ? t : (m == 1u ? -c : 0.f);
+}
+
+// MI term p log2 p, p = m/N (0 for m = 0), from v_log_f32; its error is
+// measured like k2_term's.
+__device__ __forceinline__ float plogp(uint32_t m, float invN)
+{
+#pragma clang fp contract(off)
+    const float p = (float)m * invN;
+    return m ? p * __builtin_amdgcn_logf(p) : 0.f;
+}
+
 // K2 in FP32.  Each cell contributes f = ln ctl! + ln cas! - ln (ctl+cas+1)!
-// (scoring.hpp:5-7), always <= 0, so K2 = -sum f.  The table holds
-// T'(m) = ln m! - c*m with c = ln N - 1; since cas + ctl - (cas+ctl+1) = -1
-// every cell then reads f + c, and K2 = ncells*c - sum(f + c).  The shift
-// bounds |T'| by about N/e instead of N ln N, which is what keeps the FP32
-// error small; the cell sum is Kahan-compensated.
-__device__ __forceinline__ float k2_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, const float *T,
-                                        float total)
+// (scoring.hpp:5-7), always <= 0, so K2 = -sum f.  With T'(m) = ln m! - c*m
+// and cas + ctl - (cas+ctl+1) = -1, every cell reads f + c, and
+// K2 = ncells*c - sum(f + c); the cell sum is Kahan-compensated.
+__device__ __forceinline__ float k2_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, float invN, float k0,
+                                        float c2, float total)
 {
     float s = 0.f, c = 0.f;
     for (int i = 0; i < ncells; i++) {
-        float f = (T[ctl[i]] + T[cas[i]]) - T[ctl[i] + cas[i] + 1];
+        float f = (k2_term(ctl[i], invN, k0, c2) + k2_term(cas[i], invN, k0, c2)) -
+                  k2_term(ctl[i] + cas[i] + 1, invN, k0, c2);
         kahan_add(s, c, f);
     }
     return total - s;
 }
 
 // MI in FP32: H(G) + H(Y) - H(G,Y) in bits (scoring.hpp:9-16), with
-// L(m) = log2(m/N) and compensated sums.
-__device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, const float *L,
-                                        float invN, float hY)
+// compensated sums.
+__device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, float invN, float hY)
 {
     float sG = 0.f, cG = 0.f, sGY = 0.f, cGY = 0.f;
     for (int i = 0; i < ncells; i++) {
-        float pc = (float)cas[i] * invN, pt = (float)ctl[i] * invN;
-        kahan_add(sGY, cGY, pc * L[cas[i]]);
-        kahan_add(sGY, cGY, pt * L[ctl[i]]);
-        kahan_add(sG, cG, (pc + pt) * L[cas[i] + ctl[i]]);
+        kahan_add(sGY, cGY, plogp(cas[i], invN));
+        kahan_add(sGY, cGY, plogp(ctl[i], invN));
+        kahan_add(sG, cG, plogp(cas[i] + ctl[i], invN));
     }
     // H(G) = -sG, H(G,Y) = -sGY
     return (sGY - sG) + hY;
@@ -122,8 +155,8 @@ __device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl
 // "Badness": the FP32 score mapped so that lower is always better.
 __device__ __forceinline__ float badness_f32(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl)
 {
-    if (sc.score == SCORE_K2) return k2_f32(cas, ctl, sc.ncells, sc.t32, sc.k2_total);
-    return -mi_f32(cas, ctl, sc.ncells, sc.t32, sc.invN, sc.hY);
+    if (sc.score == SCORE_K2) return k2_f32(cas, ctl, sc.ncells, sc.invN, sc.k0, sc.c, sc.k2_total);
+    return -mi_f32(cas, ctl, sc.ncells, sc.invN, sc.hY);
 }
 
 // ---------------------------------------------------------------- FP64 --

@@ -792,6 +792,31 @@ __global__ void k_init_many(unsigned long long *const *queue, unsigned *const *b
 }
 
 // Small fill kernels (no hipMemsetAsync, so the sequence stays capturable).
+// Largest error of the FP32 score terms over every argument a search can
+// reach, against the FP64 tables.  K2: |k2_term(m) - (T64[m] - c m)| for
+// m in [0, N+1].  MI: |plogp(m) - (m/N) L64[m]| * N/m for m in [1, N], the
+// error relative to p, so that a sum of terms whose p add up to P errs by at
+// most P times this.  *out holds the bits of a non-negative double, which
+// atomicMax on the bits orders correctly.
+__global__ void k_term_error(int score, int N, float invN, float k0, float c32, double c, const double *t64,
+                             unsigned long long *out)
+{
+    double e = 0.0;
+    const int hi = score == SCORE_K2 ? N + 1 : N;
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m <= hi; m += gridDim.x * blockDim.x) {
+        if (score == SCORE_K2) {
+            const double exact = t64[m] - c * (double)m;
+            e = fmax(e, fabs((double)k2_term((uint32_t)m, invN, k0, c32) - exact));
+        } else if (m >= 1) {
+            const double pm = (double)m / (double)N;
+            e = fmax(e, fabs((double)plogp((uint32_t)m, invN) - pm * t64[m]) / pm);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) e = fmax(e, __shfl_xor(e, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(e));
+}
+
 __global__ void k_init_search(unsigned long long *queue, unsigned *best_key, unsigned *cand_count,
                               unsigned long long *best_key64, unsigned long long *checksum)
 {

@@ -53,10 +53,11 @@ double now_s()
 
 struct ScoreTables {
     bool ready = false;
-    float *t32 = nullptr;
-    double *t64 = nullptr;
-    double t32_max = 0;       // max |t32| (K2)
-    double shift_c = 0;       // K2 shift per cell
+    double *t64 = nullptr;    // K2: ln m!; MI: log2(m/N) (FP64 re-scoring)
+    double t_max = 0;         // K2: max |ln m! - c m| over the table
+    double shift_c = 0;       // K2 shift per cell, c = ln N - 1
+    float invN = 0, k0 = 0, c32 = 0;   // FP32 constants of the fused scoring
+    double term_err = 0;      // measured error of the FP32 terms (k_term_error)
 };
 
 struct Ctl {                  // device control block
@@ -131,27 +132,38 @@ int ensure_tables(episdet_dataset *ds, int score)
     const int N = ds->N[0] + ds->N[1];
     const int len = N + 2;
     std::vector<double> h64(len, 0.0);
-    std::vector<float> h32(len, 0.f);
+    t.shift_c = std::log((double)N) - 1.0;
+    t.invN = (float)(1.0 / (double)N);
+    t.k0 = (float)(0.5 * std::log((double)N) - 1.0 + 0.5 * std::log(2.0 * M_PI));
+    t.c32 = (float)t.shift_c;
     if (score == EPISDET_SCORE_K2) {
-        const double c = std::log((double)N) - 1.0;
-        t.shift_c = c;
         double mx = 0;
         for (int m = 0; m < len; m++) {
             h64[m] = std::lgamma((double)m + 1.0);
-            const double v = h64[m] - c * (double)m;
-            h32[m] = (float)v;
-            mx = std::max(mx, std::fabs(v));
+            mx = std::max(mx, std::fabs(h64[m] - t.shift_c * (double)m));
         }
-        t.t32_max = mx;
+        t.t_max = mx;
     } else {
         h64[0] = 0.0;
         for (int m = 1; m <= N; m++) h64[m] = std::log2((double)m / (double)N);
-        for (int m = 0; m < len; m++) h32[m] = (float)h64[m];
     }
-    if (hipMalloc(&t.t32, sizeof(float) * len) != hipSuccess) return EPISDET_ERR_ALLOC;
     if (hipMalloc(&t.t64, sizeof(double) * len) != hipSuccess) return EPISDET_ERR_ALLOC;
-    HIPCHK(hipMemcpy(t.t32, h32.data(), sizeof(float) * len, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(t.t64, h64.data(), sizeof(double) * len, hipMemcpyHostToDevice));
+    // error of the FP32 terms over every argument this dataset can produce
+    unsigned long long *d_err = nullptr;
+    if (hipMalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) return EPISDET_ERR_ALLOC;
+    const unsigned long long zero = 0ull;
+    HIPCHK(hipMemcpy(d_err, &zero, sizeof zero, hipMemcpyHostToDevice));
+    const int blocks = std::min(1024, (len + 255) / 256);
+    hipLaunchKernelGGL(k_term_error, dim3(blocks), dim3(256), 0, ds->stream, score, N, t.invN, t.k0, t.c32, t.shift_c, t.t64,
+                       d_err);
+    HIPCHK(hipGetLastError());
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, d_err, sizeof bits, hipMemcpyDeviceToHost, ds->stream));
+    HIPCHK(hipStreamSynchronize(ds->stream));
+    (void)hipFree(d_err);
+    std::memcpy(&t.term_err, &bits, sizeof bits);
+    if (!(t.term_err >= 0.0 && t.term_err < 1e3)) return EPISDET_ERR_INTERNAL;
     t.ready = true;
     return EPISDET_OK;
 }
@@ -175,23 +187,29 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
     ScoreCtx sc;
     sc.score = score;
     sc.ncells = ncells;
-    sc.t32 = t.t32;
     sc.k2_total = 0.f;
-    sc.invN = (float)(1.0 / (double)N);
+    sc.k0 = t.k0;
+    sc.c = t.c32;
+    sc.invN = t.invN;
     const double qc = (double)ds->N[0] / N, qt = (double)ds->N[1] / N;
     sc.hY = (float)(-qc * std::log2(qc) - qt * std::log2(qt));
     double eps;
     if (score == EPISDET_SCORE_K2) {
         sc.k2_total = (float)(ncells * t.shift_c);
-        const double T = t.t32_max;
+        // per cell: three terms, each off by at most term_err, and two
+        // roundings of sums below 3T; then the compensated sum and the
+        // final subtraction
+        const double T = t.t_max + t.term_err;
         const double sum_bound = ncells * 3.0 * T;
-        eps = ncells * 3.5 * ulp32(T) + 2.0 * u * sum_bound + ulp32(sum_bound) + ulp32(ncells * t.shift_c);
+        eps = ncells * (3.0 * t.term_err + 2.0 * ulp32(3.0 * T)) + 2.0 * u * sum_bound + ulp32(sum_bound) +
+              ulp32(ncells * t.shift_c);
     } else {
-        // Terms p*log2(p) are bounded by 0.531 in magnitude; the entropies by
-        // log2(2*ncells).  Each of the 3*ncells products carries a few
-        // roundings; the compensated sums add about one ulp of the result.
+        // Terms p log2 p: the probabilities of each of H(G,Y) and H(G) add up
+        // to 1, so the terms err by at most 2 term_err in total; the
+        // compensated sums and the final additions add a few ulps of values
+        // bounded by log2(2 ncells).
         const double hmax = std::log2(2.0 * ncells);
-        eps = u * (3.0 * ncells * 0.531 * 6.0 + 8.0 * hmax) + 4.0 * u;
+        eps = 2.0 * t.term_err + u * (3.0 * ncells * 0.531 * 2.0 + 8.0 * hmax) + 4.0 * u;
     }
     sc.margin = (float)(2.0 * eps);
     return sc;
@@ -959,7 +977,6 @@ void episdet_dataset_destroy(episdet_dataset *ds)
     (void)hipDeviceSynchronize();
     for (int c = 0; c < 2; c++) {
         if (ds->planes[c]) (void)hipFree(ds->planes[c]);
-        if (ds->tab[c].t32) (void)hipFree(ds->tab[c].t32);
         if (ds->tab[c].t64) (void)hipFree(ds->tab[c].t64);
     }
     if (ds->singles) (void)hipFree(ds->singles);
