@@ -90,7 +90,7 @@ typedef struct {
     int order;            /* 2 or 3                                              */
     int score;            /* episdet_score                                       */
     int path;             /* episdet_path                                        */
-    int tile;             /* 0 = autotuned; otherwise a kernel tile id           */
+    int tile;             /* kernel tile id of the chosen path (ignored by AUTO)  */
     int plain_rounds;     /* PLAIN only: stop after this many rounds (0 = all);
                              used to time a contiguous slice of the round loop   */
     int candidate_cap;    /* 0 = default (1<<20)                                 */
@@ -100,7 +100,11 @@ typedef struct {
                              search-triplets.cu:113-117,173-197)                 */
     int topk;             /* > 1: keep the topk best sets (episdet_run_topk);
                              0 or 1: the single best set                         */
-    int reserved[6];
+    int autotune;         /* AUTO only: 1 = time the candidate kernels for this
+                             shape first if it has no cached choice
+                             (episdet_autotune); 0 = cached choice or the
+                             default rule                                        */
+    int reserved[5];
 } episdet_options;
 
 typedef struct {
@@ -130,6 +134,28 @@ typedef struct {
 } episdet_result;
 
 int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result *res);
+
+/* ---- autotuner (no reference counterpart: the reference's BLOCK_SIZE and
+ * NUM_STREAMS are compile-time constants, search-triplets.cu:4-5) ----
+ *
+ * Times every kernel configuration that can run this dataset's shape
+ * (FP4-MFMA tiles, VALU tiles) on a window of work units from the middle of
+ * its work queue, predicts each one's whole-search kernel time from the exact
+ * number of sets in the window, and caches the fastest for the shape
+ * (order, score, n, words per class) for the rest of the process: AUTO then
+ * runs it, in episdet_run and in episdet_run_batch.  entries: up to
+ * capacity of them, in the order timed; *count receives the number of
+ * configurations timed, *chosen the index of the one cached. */
+typedef struct {
+    int path;               /* EPISDET_PATH_FUSED_VALU or _FUSED_MFMA           */
+    int tile;               /* tile id                                          */
+    double est_ms;          /* predicted kernel time of the whole search        */
+    double sample_ms;       /* measured kernel time of the window               */
+    long long sample_sets;  /* sets in the window                               */
+} episdet_tune_entry;
+
+int episdet_autotune(episdet_dataset *ds, int order, int score, episdet_tune_entry *entries, int capacity,
+                     int *count, int *chosen);
 
 /* The k best sets of one search, by exact FP64 score: K2 ascending or MI
  * descending, ties to the lowest packed index.  The reference keeps only the

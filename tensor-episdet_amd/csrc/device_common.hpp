@@ -15,7 +15,7 @@ struct ScoreCtx {
     int ncells;         // 9 or 27
     float k2_total;     // K2: ncells * c, c = ln N - 1
     float k0;           // K2: ln(N)/2 - 1 + ln(2 pi)/2 (see k2_term)
-    float c;            // K2: c = ln N - 1
+    float c;            // K2: ln N - 1
     float invN;         // 1/N
     float hY;           // MI: H(Y) in bits
     float margin;       // candidate margin, 2 x the FP32 error bound
@@ -89,28 +89,34 @@ __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
 
 constexpr float LN2_F = 0.693147180559945309f;
 
-// K2 term T'(m) = ln m! - c*m, c = ln N - 1, computed without memory access
-// from Stirling's series for ln Gamma(x), x = m + 1, with the shift folded
-// in:
-//   T'(m) = (m + 1/2) ln(x/N) + K0 + 1/(12x) - 1/(360x^3) + 1/(1260x^5),
-//   K0 = ln(N)/2 - 1 + ln(2 pi)/2,
-// and exactly T'(0) = 0, T'(1) = -c.  The truncation error is below
-// 1/(1680x^7) (5e-6 at m = 2); ln(x/N) comes from v_log_f32.  Writing the
-// shift into ln(x/N) keeps every intermediate near |T'| <= N/e instead of
-// N ln N, so the FP32 error stays about an ulp of |T'|.  The host measures
-// the error of exactly this code against the FP64 table over every m a
-// search can reach (k_term_error) and widens the candidate margin by it.
-// No contraction, so every instantiation rounds identically.
-__device__ __forceinline__ float k2_term(uint32_t m, float invN, float k0, float c)
+// ln m! for m < LNF_SMALL in FP32, set once per device by the host; each
+// scoring workgroup copies it to LDS (load_lnf).
+constexpr int LNF_SMALL = 1024;
+__device__ float g_lnfact[LNF_SMALL];
+
+__device__ __forceinline__ void load_lnf(float *s_lnf)
+{
+    for (int i = threadIdx.x; i < LNF_SMALL; i += blockDim.x) s_lnf[i] = g_lnfact[i];
+}
+
+// K2 term T'(m) = ln m! - c*m, c = ln N - 1.
+//  * m < LNF_SMALL: ln m! from the LDS copy of g_lnfact, shifted with one fma.
+//  * larger m: Stirling's series with the shift folded into the logarithm,
+//      T'(m) = (x - 1/2) ln(x/N) + K0,   x = m + 1,  K0 = ln(N)/2 - 1 + ln(2 pi)/2,
+//    truncated after the constant (the next term, 1/(12x), is below 8.2e-5
+//    here); ln(x/N) comes from v_log_f32.  The shift inside the logarithm
+//    keeps every intermediate near |T'| <= N/e instead of N ln N.
+// No table gather from global memory, one transcendental at most.  The host
+// measures the error of exactly this code against the FP64 table over every
+// m a search can reach (k_term_error) and widens the candidate margin by it.
+// Explicit fmaf and no contraction, so every instantiation rounds the same.
+__device__ __forceinline__ float k2_term(uint32_t m, float invN, float k0, float c, const float *s_lnf)
 {
 #pragma clang fp contract(off)
+    if (m < (uint32_t)LNF_SMALL) return fmaf(-c, (float)m, s_lnf[m]);
     const float x = (float)(m + 1u);
     const float l2 = __builtin_amdgcn_logf(x * invN);
-    const float ix = __builtin_amdgcn_rcpf(x);
-    const float ix2 = ix * ix;
-    const float corr = ix * (1.f / 12.f - ix2 * (1.f / 360.f - ix2 * (1.f / 1260.f)));
-    const float t = (((float)m + 0.5f) * LN2_F) * l2 + (k0 + corr);
-    return m > 1u ? t : (m == 1u ? -c : 0.f);
+    return fmaf(fmaf(x, LN2_F, -0.5f * LN2_F), l2, k0);
 }
 
 // MI term p log2 p, p = m/N (0 for m = 0), from v_log_f32; its error is
@@ -127,12 +133,12 @@ __device__ __forceinline__ float plogp(uint32_t m, float invN)
 // and cas + ctl - (cas+ctl+1) = -1, every cell reads f + c, and
 // K2 = ncells*c - sum(f + c); the cell sum is Kahan-compensated.
 __device__ __forceinline__ float k2_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, float invN, float k0,
-                                        float c2, float total)
+                                        float c2, float total, const float *s_lnf)
 {
     float s = 0.f, c = 0.f;
     for (int i = 0; i < ncells; i++) {
-        float f = (k2_term(ctl[i], invN, k0, c2) + k2_term(cas[i], invN, k0, c2)) -
-                  k2_term(ctl[i] + cas[i] + 1, invN, k0, c2);
+        float f = (k2_term(ctl[i], invN, k0, c2, s_lnf) + k2_term(cas[i], invN, k0, c2, s_lnf)) -
+                  k2_term(ctl[i] + cas[i] + 1, invN, k0, c2, s_lnf);
         kahan_add(s, c, f);
     }
     return total - s;
@@ -153,9 +159,10 @@ __device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl
 }
 
 // "Badness": the FP32 score mapped so that lower is always better.
-__device__ __forceinline__ float badness_f32(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl)
+__device__ __forceinline__ float badness_f32(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl,
+                                            const float *s_lnf)
 {
-    if (sc.score == SCORE_K2) return k2_f32(cas, ctl, sc.ncells, sc.invN, sc.k0, sc.c, sc.k2_total);
+    if (sc.score == SCORE_K2) return k2_f32(cas, ctl, sc.ncells, sc.invN, sc.k0, sc.c, sc.k2_total, s_lnf);
     return -mi_f32(cas, ctl, sc.ncells, sc.invN, sc.hY);
 }
 

@@ -19,7 +19,7 @@
 //    four.  Output lane l, register r holds C[8(r/4)+4(l/32)+r%4][l%32].
 //
 // Workgroup: 8 waves on a 32 Y x 64 Z tile (the same work units as the VALU
-// kernel's (2,4) tile).  Waves 0-3 contract the cases, waves 4-7 the
+// kernel's (2,4) tile); it takes a batch of jobs like version 2 below.  Waves 0-3 contract the cases, waves 4-7 the
 // controls; wave (wy, wz) of a class owns 16 Y x 32 Z = 2 x 2 MFMA tiles, so
 // each wave holds 64 accumulators.  Each half stages its own class: P and Z
 // rows of a 2048-sample chunk go to LDS in row-major [row][68] layout
@@ -133,30 +133,52 @@ __device__ __forceinline__ void mf_step(const uint4 (&a)[2], const uint4 (&b)[2]
     acc[1][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A1, B1, acc[1][1], 4, 4, 0, s, 0, s);
 }
 
+// A batch of independent 3-way searches sharing one work queue: units
+// [job_start[j], job_start[j+1]) belong to job j.  A single search is a batch
+// of one.
+struct BatchTri {
+    const TriArgs *jobs;                    // device array, one per job
+    const unsigned long long *job_start;    // [njobs + 1]
+    int njobs;
+    unsigned long long total;
+    unsigned long long *queue;
+};
+
 template <int MODE>
-__global__ __launch_bounds__(512) void k_triplets_mfma(TriArgs p)
+__global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
 {
     __shared__ __attribute__((aligned(16))) uint32_t smem[MF_SMEM_WORDS];
     __shared__ unsigned long long s_u;
     __shared__ float s_red[16];
+    __shared__ float s_lnf[scores_sets(MODE) ? LNF_SMALL : 1];
+    if constexpr (scores_sets(MODE)) load_lnf(s_lnf);
     const int tid = threadIdx.x;
     const int cls = tid >> 8, t = tid & 255;            // class half and thread within it
     const int wih = t >> 6, lane = t & 63;              // wave within the half
     const int wy = wih >> 1, wz = wih & 1, h = lane >> 5, i = lane & 31;
     uint32_t (*rows)[MF_LDSW] = (uint32_t (*)[MF_LDSW])(smem + cls * MF_ROWS * MF_LDSW);
-    const uint32_t *base = cls ? p.p1 : p.p0;
-    const int W = cls ? p.W1 : p.W0;
-    const int nch_mine = W / MF_KW;
-    const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / MF_KW;
     const int arow0 = (wy * 16) * 4 + i, arow1 = (wy * 16 + 8) * 4 + i;
     const int brow0 = MF_TY * 4 + (wz * 32) * 2 + i, brow1 = MF_TY * 4 + (wz * 32 + 16) * 2 + i;
 
-    int g = 0;
+    int g = 0, j = 0;
     for (;;) {
-        if (tid == 0) s_u = atomicAdd(p.queue, 1ull);
+        if (tid == 0) s_u = atomicAdd(bt.queue, 1ull);
         __syncthreads();
-        const unsigned long long u = s_u;
-        if (u >= p.nunits) break;
+        const unsigned long long uu = s_u;
+        if (uu >= bt.total) break;
+        // job of this unit; a workgroup's units only increase
+        int jn = j;
+        while (jn + 1 < bt.njobs && bt.job_start[jn + 1] <= uu) jn++;
+        if (jn != j) {
+            j = jn;
+            g = 0;
+        }
+        const TriArgs &p = bt.jobs[j];
+        const unsigned long long u = uu - bt.job_start[j];
+        const uint32_t *base = cls ? p.p1 : p.p0;
+        const int W = cls ? p.W1 : p.W0;
+        const int nch_mine = W / MF_KW;
+        const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / MF_KW;
         g = next_group(p.gstart, p.ngroups, u, g);
         const int X = (int)(u - p.gstart[g]);
         const int2 gt = p.groups[g];
@@ -248,7 +270,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(TriArgs p)
             uint32_t cas[27], ctl[27];
             triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
             if constexpr (scores_sets(MODE)) {
-                bad[q4] = badness_f32(p.sc, cas, ctl);
+                bad[q4] = badness_f32(p.sc, cas, ctl, s_lnf);
             } else if constexpr (MODE == MODE_DUMP) {
                 uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
                 for (int c = 0; c < 27; c++) {
@@ -364,16 +386,6 @@ __device__ __forceinline__ void m2_step(const uint4 &xw, const uint4 (&yw)[2], c
     }
 }
 
-// A batch of independent 3-way searches sharing one work queue: units
-// [job_start[j], job_start[j+1]) belong to job j.  A single search is a batch
-// of one.
-struct BatchTri {
-    const TriArgs *jobs;                    // device array, one per job
-    const unsigned long long *job_start;    // [njobs + 1]
-    int njobs;
-    unsigned long long total;
-    unsigned long long *queue;
-};
 
 template <int MODE>
 __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
@@ -381,6 +393,8 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
     __shared__ __attribute__((aligned(16))) uint32_t smem[M2_SMEM_WORDS];
     __shared__ unsigned long long s_u;
     __shared__ float s_red[16];
+    __shared__ float s_lnf[scores_sets(MODE) ? LNF_SMALL : 1];
+    if constexpr (scores_sets(MODE)) load_lnf(s_lnf);
     const int tid = threadIdx.x;
     const int cls = tid >> 8, t = tid & 255;
     const int wy = t >> 6, lane = t & 63, h = lane >> 5, i = lane & 31;
@@ -494,7 +508,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {
-                    bad[m] = badness_f32(p.sc, cas, ctl);
+                    bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                 } else if constexpr (MODE == MODE_DUMP) {
                     uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
                     for (int c = 0; c < 27; c++) {

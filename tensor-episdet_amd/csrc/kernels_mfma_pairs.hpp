@@ -110,6 +110,8 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
     __shared__ __attribute__((aligned(16))) uint32_t smem[P2_SMEM_WORDS];
     __shared__ unsigned long long s_u;
     __shared__ float s_red[16];
+    __shared__ float s_lnf[scores_sets(MODE) ? LNF_SMALL : 1];
+    if constexpr (scores_sets(MODE)) load_lnf(s_lnf);
     const int tid = threadIdx.x;
     const int cls = tid >> 8, t = tid & 255;
     const int wy = t >> 6, lane = t & 63, h = lane >> 5, i = lane & 31;
@@ -225,7 +227,7 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
                     pair9(n4c[0], n4c[1], n4c[2], n4c[3], sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
                     pair9(n4t[0], n4t[1], n4t[2], n4t[3], sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);
                     if constexpr (scores_sets(MODE)) {
-                        bad[m] = badness_f32(p.sc, cas, ctl);
+                        bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {
                         uint32_t *o = p.dump + rank2(A, B) * 18;
                         for (int c = 0; c < 9; c++) {

@@ -294,6 +294,8 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
     __shared__ uint32_t sB[2][T][LDSW];
     __shared__ unsigned long long s_u;
     __shared__ float s_red[8];
+    __shared__ float s_lnf[scores_sets(MODE) ? LNF_SMALL : 1];
+    if constexpr (scores_sets(MODE)) load_lnf(s_lnf);
     const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
     int j = 0;
     for (;;) {
@@ -336,7 +338,7 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
                     pair9(acc1[r][s][0], acc1[r][s][1], acc1[r][s][2], acc1[r][s][3], sa.z, sa.w, sb.z, sb.w,
                           (uint32_t)p.N1, ctl);
                     if constexpr (scores_sets(MODE)) {
-                        bad[m] = badness_f32(p.sc, cas, ctl);
+                        bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {
                         uint32_t *o = p.dump + rank2(A, B) * 18;
                         for (int c = 0; c < 9; c++) {
@@ -512,6 +514,8 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
     __shared__ uint32_t sZ[2][TZ][LDSW];
     __shared__ unsigned long long s_u;
     __shared__ float s_red[8];
+    __shared__ float s_lnf[scores_sets(MODE) ? LNF_SMALL : 1];
+    if constexpr (scores_sets(MODE)) load_lnf(s_lnf);
     const int tid = threadIdx.x, ty = tid >> 4, tz = tid & 15;
     int g = 0;
     for (int iter = 0;; iter++) {
@@ -570,7 +574,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
                     uint32_t cas[27], ctl[27];
                     triplet_tables(p, X, Y, Z, sx, sy, acc0[a][b], acc1[a][b], cas, ctl);
                     if constexpr (scores_sets(MODE)) {
-                        bad[m] = badness_f32(p.sc, cas, ctl);
+                        bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {
                         uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
                         for (int c = 0; c < 27; c++) {
@@ -791,7 +795,6 @@ __global__ void k_init_many(unsigned long long *const *queue, unsigned *const *b
     if (i < nq) batch_queues[i] = 0ull;
 }
 
-// Small fill kernels (no hipMemsetAsync, so the sequence stays capturable).
 // Largest error of the FP32 score terms over every argument a search can
 // reach, against the FP64 tables.  K2: |k2_term(m) - (T64[m] - c m)| for
 // m in [0, N+1].  MI: |plogp(m) - (m/N) L64[m]| * N/m for m in [1, N], the
@@ -801,12 +804,15 @@ __global__ void k_init_many(unsigned long long *const *queue, unsigned *const *b
 __global__ void k_term_error(int score, int N, float invN, float k0, float c32, double c, const double *t64,
                              unsigned long long *out)
 {
+    __shared__ float s_lnf[LNF_SMALL];
+    load_lnf(s_lnf);
+    __syncthreads();
     double e = 0.0;
     const int hi = score == SCORE_K2 ? N + 1 : N;
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m <= hi; m += gridDim.x * blockDim.x) {
         if (score == SCORE_K2) {
             const double exact = t64[m] - c * (double)m;
-            e = fmax(e, fabs((double)k2_term((uint32_t)m, invN, k0, c32) - exact));
+            e = fmax(e, fabs((double)k2_term((uint32_t)m, invN, k0, c32, s_lnf) - exact));
         } else if (m >= 1) {
             const double pm = (double)m / (double)N;
             e = fmax(e, fabs((double)plogp((uint32_t)m, invN) - pm * t64[m]) / pm);
@@ -817,6 +823,7 @@ __global__ void k_term_error(int score, int N, float invN, float k0, float c32, 
     if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(e));
 }
 
+// Small fill kernels (no hipMemsetAsync, so the sequence stays capturable).
 __global__ void k_init_search(unsigned long long *queue, unsigned *best_key, unsigned *cand_count,
                               unsigned long long *best_key64, unsigned long long *checksum)
 {

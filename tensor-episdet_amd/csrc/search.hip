@@ -17,6 +17,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#include <tuple>
+#include <mutex>
+#include <map>
 
 #include "episdet.h"
 #include "kernels_mfma.hpp"
@@ -56,7 +59,7 @@ struct ScoreTables {
     double *t64 = nullptr;    // K2: ln m!; MI: log2(m/N) (FP64 re-scoring)
     double t_max = 0;         // K2: max |ln m! - c m| over the table
     double shift_c = 0;       // K2 shift per cell, c = ln N - 1
-    float invN = 0, k0 = 0, c32 = 0;   // FP32 constants of the fused scoring
+    float invN = 0, k0 = 0;   // FP32 constants of the fused scoring
     double term_err = 0;      // measured error of the FP32 terms (k_term_error)
 };
 
@@ -125,17 +128,32 @@ int ensure_cand(episdet_dataset *ds, unsigned cap)
 
 // Host tables.  FP64: exactly the oracle's (lgamma / log2 of glibc), so FP64
 // scores agree bit for bit.  FP32: the shifted K2 table or the MI table.
+// ln m! for the small K2 terms (g_lnfact), once per process; glibc lgamma in
+// FP64, rounded to FP32.
+int ensure_lnfact()
+{
+    static std::mutex mu;
+    static bool done = false;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done) return EPISDET_OK;
+    std::vector<float> h(LNF_SMALL);
+    for (int m = 0; m < LNF_SMALL; m++) h[m] = (float)std::lgamma((double)m + 1.0);
+    HIPCHK(hipMemcpyToSymbol(HIP_SYMBOL(g_lnfact), h.data(), sizeof(float) * LNF_SMALL));
+    done = true;
+    return EPISDET_OK;
+}
+
 int ensure_tables(episdet_dataset *ds, int score)
 {
     ScoreTables &t = ds->tab[score];
     if (t.ready) return EPISDET_OK;
+    CHK(ensure_lnfact());
     const int N = ds->N[0] + ds->N[1];
     const int len = N + 2;
     std::vector<double> h64(len, 0.0);
     t.shift_c = std::log((double)N) - 1.0;
     t.invN = (float)(1.0 / (double)N);
     t.k0 = (float)(0.5 * std::log((double)N) - 1.0 + 0.5 * std::log(2.0 * M_PI));
-    t.c32 = (float)t.shift_c;
     if (score == EPISDET_SCORE_K2) {
         double mx = 0;
         for (int m = 0; m < len; m++) {
@@ -155,7 +173,7 @@ int ensure_tables(episdet_dataset *ds, int score)
     const unsigned long long zero = 0ull;
     HIPCHK(hipMemcpy(d_err, &zero, sizeof zero, hipMemcpyHostToDevice));
     const int blocks = std::min(1024, (len + 255) / 256);
-    hipLaunchKernelGGL(k_term_error, dim3(blocks), dim3(256), 0, ds->stream, score, N, t.invN, t.k0, t.c32, t.shift_c, t.t64,
+    hipLaunchKernelGGL(k_term_error, dim3(blocks), dim3(256), 0, ds->stream, score, N, t.invN, t.k0, (float)t.shift_c, t.shift_c, t.t64,
                        d_err);
     HIPCHK(hipGetLastError());
     unsigned long long bits = 0;
@@ -189,7 +207,7 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
     sc.ncells = ncells;
     sc.k2_total = 0.f;
     sc.k0 = t.k0;
-    sc.c = t.c32;
+    sc.c = (float)t.shift_c;
     sc.invN = t.invN;
     const double qc = (double)ds->N[0] / N, qt = (double)ds->N[1] / N;
     sc.hY = (float)(-qc * std::log2(qc) - qt * std::log2(qt));
@@ -276,18 +294,15 @@ int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
     const void *k = tile == 1 ? (const void *)k_triplets_mfma<MODE> : (const void *)k_triplets_mfma2<MODE>;
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 512, 0) != hipSuccess || per < 1) per = 1;
-    if (tile == 1) {
-        hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, a);
-    } else {
-        CHK(upload_single(ds, a, a.nunits));
-        BatchTri bt;
-        bt.jobs = (const TriArgs *)ds->jobargs;
-        bt.job_start = ds->jobstart;
-        bt.njobs = 1;
-        bt.total = a.nunits;
-        bt.queue = a.queue;
-        hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(per * ds->cu), dim3(512), 0, s, bt);
-    }
+    CHK(upload_single(ds, a, a.nunits));
+    BatchTri bt;
+    bt.jobs = (const TriArgs *)ds->jobargs;
+    bt.job_start = ds->jobstart;
+    bt.njobs = 1;
+    bt.total = a.nunits;
+    bt.queue = a.queue;
+    if (tile == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, bt);
+    else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(per * ds->cu), dim3(512), 0, s, bt);
     HIPCHK(hipGetLastError());
     return EPISDET_OK;
 }
@@ -567,12 +582,311 @@ int collect_topk(episdet_dataset *ds, int order, int score, int count, int k, ep
     return EPISDET_OK;
 }
 
-// AUTO resolves to the FP4-MFMA kernels wherever they are exact (fewer than
-// 2^24 samples per class), else to the VALU kernels.
-int resolve_path(const episdet_dataset *ds, const episdet_options *o)
+
+// One fused kernel configuration for one dataset and search: the argument
+// block and, for 3-way, the work-unit groups (host copy kept for counting
+// the sets of a unit window).
+struct FusedPlan {
+    int order = 0, score = 0, path = 0, tile = 0;
+    bool mfma = false;
+    TriArgs ta;
+    PairArgs pa;
+    ScoreCtx sc;
+    std::vector<int2> groups;
+    std::vector<unsigned long long> gstart;
+    unsigned long long *d_gstart = nullptr;
+    int2 *d_groups = nullptr;
+    unsigned long long nunits = 0;
+    FusedPlan() = default;
+    FusedPlan(const FusedPlan &) = delete;
+    FusedPlan &operator=(const FusedPlan &) = delete;
+    ~FusedPlan()
+    {
+        if (d_gstart) (void)hipFree(d_gstart);
+        if (d_groups) (void)hipFree(d_groups);
+    }
+};
+
+bool tile_valid(int order, int path, int tile)
 {
-    if (o->path != EPISDET_PATH_AUTO) return o->path;
-    return mfma_ok(ds) ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
+    if (tile < 0) return false;
+    if (path == EPISDET_PATH_FUSED_MFMA) return tile < (order == 3 ? MFMA_TILES : 1);
+    if (order == 3) return tile < (int)(sizeof TRI_TILES / sizeof TRI_TILES[0]);
+    return tile < (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0]);
+}
+
+// Tables, pair table and candidate buffer must exist (ensure_*) before.
+int plan_fused(episdet_dataset *ds, int order, int score, int path, int tile, FusedPlan &pl)
+{
+    pl.order = order;
+    pl.score = score;
+    pl.path = path;
+    pl.tile = tile;
+    pl.mfma = path == EPISDET_PATH_FUSED_MFMA;
+    if (pl.mfma && !mfma_ok(ds)) return EPISDET_ERR_INVALID;
+    if (!tile_valid(order, path, tile)) return EPISDET_ERR_INVALID;
+    pl.sc = make_ctx(ds, score, order);
+    if (order == 3) {
+        if (pl.mfma) {
+            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), pl.groups, pl.gstart);
+        } else {
+            const Tri3Tile t = TRI_TILES[tile];
+            build_groups(ds->n, 16 * t.A, 16 * t.B, pl.groups, pl.gstart);
+        }
+        if (pl.groups.empty()) return EPISDET_ERR_INVALID;
+        if (hipMalloc(&pl.d_gstart, sizeof(unsigned long long) * pl.gstart.size()) != hipSuccess)
+            return EPISDET_ERR_ALLOC;
+        if (hipMalloc(&pl.d_groups, sizeof(int2) * pl.groups.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
+        HIPCHK(hipMemcpy(pl.d_gstart, pl.gstart.data(), sizeof(unsigned long long) * pl.gstart.size(),
+                         hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(pl.d_groups, pl.groups.data(), sizeof(int2) * pl.groups.size(), hipMemcpyHostToDevice));
+        fill_tri_common(ds, pl.ta);
+        pl.ta.sc = pl.sc;
+        pl.ta.gstart = pl.d_gstart;
+        pl.ta.groups = pl.d_groups;
+        pl.ta.ngroups = (int)pl.groups.size();
+        pl.nunits = pl.gstart.back();
+        pl.ta.nunits = pl.nunits;
+    } else {
+        fill_pairs(ds, pl.pa, pl.mfma, tile);
+        pl.pa.sc = pl.sc;
+        pl.nunits = pl.pa.nunits;
+    }
+    return EPISDET_OK;
+}
+
+// Launches the plan over work units [u0, u1) (the whole search is
+// [0, nunits)) and returns the kernel time; the control block is reset and
+// the queue starts at u0.
+int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long long u0, unsigned long long u1,
+                 float *ms)
+{
+    TriArgs &ta = pl.ta;
+    PairArgs &pa = pl.pa;
+    ta.co.cand = pa.co.cand = ds->cand;
+    ta.co.cand_s32 = pa.co.cand_s32 = ds->cand_s32;
+    ta.co.cand_cap = pa.co.cand_cap = ds->cand_cap;
+    ta.co.topk = pa.co.topk = (unsigned)topk;
+    ta.nunits = pa.nunits = u1;
+    CHK(init_ctl(ds));
+    if (u0) HIPCHK(hipMemcpyAsync(&ds->ctl->queue, &u0, sizeof u0, hipMemcpyHostToDevice, ds->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return EPISDET_ERR_HIP;
+    }
+    (void)hipEventRecord(e0, ds->stream);
+    const int order = pl.order, tile = pl.tile;
+    const bool mfma = pl.mfma;
+    int rc;
+    if (topk)
+        rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_TOPK>(ds, tile, ta, ds->stream)
+                                : dispatch_tri<MODE_TOPK>(ds, tile, ta, 0, ds->stream))
+                        : dispatch_pairs<MODE_TOPK>(ds, mfma, tile, pa, ds->stream);
+    else
+        rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream)
+                                : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
+                        : dispatch_pairs<MODE_SEARCH>(ds, mfma, tile, pa, ds->stream);
+    (void)hipEventRecord(e1, ds->stream);
+    const bool synced = hipEventSynchronize(e1) == hipSuccess;
+    float k = 0;
+    if (synced) (void)hipEventElapsedTime(&k, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ta.nunits = pa.nunits = pl.nunits;
+    if (rc == EPISDET_OK && !synced) rc = EPISDET_ERR_HIP;
+    *ms = k;
+    return rc;
+}
+
+int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune);
+
+// ------------------------------------------------------------ autotuner --
+//
+// For each shape (order, score, n, W0, W1) the tuner times every kernel
+// configuration that can run it on a window of work units from the middle
+// of its queue (three waves of resident workgroups, or the whole search if
+// it is smaller), counts the sets in that window exactly, and predicts the
+// whole search as time x all sets / window sets.  The fastest prediction is
+// cached for the process and used by AUTO.
+
+struct TuneKey {
+    int order, score, n, W0, W1;
+    bool operator<(const TuneKey &o) const
+    {
+        return std::tie(order, score, n, W0, W1) < std::tie(o.order, o.score, o.n, o.W0, o.W1);
+    }
+};
+struct TuneChoice {
+    int path, tile;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneChoice> g_tune;
+
+void host_unrank_tri(unsigned long long u, long long &ta, long long &tb)
+{
+    long long b = (long long)((std::sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
+    while ((unsigned long long)(b + 1) * (unsigned long long)(b + 2) / 2 <= u) b++;
+    while ((unsigned long long)b * (unsigned long long)(b + 1) / 2 > u) b--;
+    tb = b;
+    ta = (long long)(u - (unsigned long long)b * (unsigned long long)(b + 1) / 2);
+}
+
+void host_unrank_p2(unsigned long long u, long long &ta, long long &tb)
+{
+    long long m = (long long)((std::sqrt(4.0 * (double)u + 1.0) - 1.0) * 0.5);
+    while ((unsigned long long)(m + 1) * (unsigned long long)(m + 2) <= u) m++;
+    while ((unsigned long long)m * (unsigned long long)(m + 1) > u) m--;
+    const unsigned long long e = (unsigned long long)m * (unsigned long long)(m + 1);
+    if (u < e + (unsigned long long)(m + 1)) {
+        tb = 2 * m;
+        ta = (long long)(u - e);
+    } else {
+        tb = 2 * m + 1;
+        ta = (long long)(u - (unsigned long long)(m + 1) * (unsigned long long)(m + 1));
+    }
+}
+
+// Valid sets (X < Y < Z < n, or A < B < n) in work units [u0, u1) of a plan.
+unsigned long long count_sets(const episdet_dataset *ds, const FusedPlan &pl, unsigned long long u0,
+                              unsigned long long u1)
+{
+    const long long n = ds->n;
+    unsigned long long total = 0;
+    if (pl.order == 3) {
+        const long long TY = pl.mfma ? mfma_ty(pl.tile) : 16 * TRI_TILES[pl.tile].A;
+        const long long TZ = pl.mfma ? mfma_tz(pl.tile) : 16 * TRI_TILES[pl.tile].B;
+        size_t g = (size_t)(std::upper_bound(pl.gstart.begin(), pl.gstart.end(), u0) - pl.gstart.begin()) - 1;
+        for (unsigned long long u = u0; u < u1; u++) {
+            while (g + 1 < pl.groups.size() && pl.gstart[g + 1] <= u) g++;
+            const long long X = (long long)(u - pl.gstart[g]);
+            const long long ylo = (long long)pl.groups[g].x * TY, zlo = (long long)pl.groups[g].y * TZ;
+            const long long zhi = std::min(zlo + TZ, n);
+            for (long long Y = std::max(ylo, X + 1); Y < std::min(ylo + TY, n); Y++) {
+                const long long z0 = std::max(zlo, Y + 1);
+                if (zhi > z0) total += (unsigned long long)(zhi - z0);
+            }
+        }
+    } else {
+        const long long TA = pl.mfma ? P2_TA : 16 * PAIR_TILES[pl.tile];
+        const long long TB = pl.mfma ? P2_TB : 16 * PAIR_TILES[pl.tile];
+        for (unsigned long long u = u0; u < u1; u++) {
+            long long ta, tb;
+            if (pl.mfma) host_unrank_p2(u, ta, tb);
+            else host_unrank_tri(u, ta, tb);
+            const long long b0 = tb * TB, bhi = std::min(b0 + TB, n);
+            for (long long A = ta * TA; A < std::min(ta * TA + TA, n); A++) {
+                const long long blo = std::max(b0, A + 1);
+                if (bhi > blo) total += (unsigned long long)(bhi - blo);
+            }
+        }
+    }
+    return total;
+}
+
+int occupancy_blocks(const void *k, int threads, int cu)
+{
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, threads, 0) != hipSuccess || per < 1) per = 1;
+    return per * cu;
+}
+
+// Workgroups a plan's search launch keeps resident.
+unsigned long long resident_blocks(const episdet_dataset *ds, const FusedPlan &pl)
+{
+    const int cu = ds->cu;
+    if (pl.order == 3) {
+        if (pl.mfma)
+            return (unsigned long long)occupancy_blocks(pl.tile == 1 ? (const void *)k_triplets_mfma<MODE_SEARCH>
+                                                                     : (const void *)k_triplets_mfma2<MODE_SEARCH>,
+                                                        512, cu);
+        const Tri3Tile t = TRI_TILES[pl.tile];
+        if (t.A == 2 && t.B == 2) return (unsigned long long)launch_blocks((const void *)k_triplets_valu<2, 2, MODE_SEARCH>, cu);
+        if (t.A == 4 && t.B == 2) return (unsigned long long)launch_blocks((const void *)k_triplets_valu<4, 2, MODE_SEARCH>, cu);
+        return (unsigned long long)launch_blocks((const void *)k_triplets_valu<2, 4, MODE_SEARCH>, cu);
+    }
+    if (pl.mfma) return (unsigned long long)occupancy_blocks((const void *)k_pairs_mfma<MODE_SEARCH>, 512, cu);
+    if (PAIR_TILES[pl.tile] == 2) return (unsigned long long)launch_blocks((const void *)k_pairs_valu<2, MODE_SEARCH>, cu);
+    return (unsigned long long)launch_blocks((const void *)k_pairs_valu<4, MODE_SEARCH>, cu);
+}
+
+// Times every configuration for this dataset's shape; fills `table` (may be
+// null) and returns the fastest in path / tile.
+int tune_kernel(episdet_dataset *ds, int order, int score, std::vector<episdet_tune_entry> *table, int *path,
+                int *tile)
+{
+    std::vector<std::pair<int, int>> cands;
+    if (mfma_ok(ds)) {
+        cands.push_back({EPISDET_PATH_FUSED_MFMA, 0});
+        if (order == 3) cands.push_back({EPISDET_PATH_FUSED_MFMA, 1});
+    }
+    if (order == 3) {
+        cands.push_back({EPISDET_PATH_FUSED_VALU, 0});
+        cands.push_back({EPISDET_PATH_FUSED_VALU, 1});
+        cands.push_back({EPISDET_PATH_FUSED_VALU, 3});
+    } else {
+        cands.push_back({EPISDET_PATH_FUSED_VALU, 0});
+        cands.push_back({EPISDET_PATH_FUSED_VALU, 1});
+    }
+    CHK(ensure_tables(ds, score));
+    if (order == 3) CHK(ensure_pairs(ds));
+    CHK(ensure_cand(ds, DEFAULT_CAP));
+    const double all_sets = (double)episdet_n_choose_k((unsigned)ds->n, (unsigned)order);
+    double best = INFINITY;
+    for (const auto &c : cands) {
+        FusedPlan pl;
+        CHK(plan_fused(ds, order, score, c.first, c.second, pl));
+        const unsigned long long k = std::min<unsigned long long>(pl.nunits, 3 * resident_blocks(ds, pl));
+        const unsigned long long u0 = (pl.nunits - k) / 2;
+        const unsigned long long sets = count_sets(ds, pl, u0, u0 + k);
+        float ms = 0;
+        CHK(launch_fused(ds, pl, 0, u0, u0 + k, &ms));    // first launch loads the code object
+        CHK(launch_fused(ds, pl, 0, u0, u0 + k, &ms));
+        const double est = sets ? (double)ms * all_sets / (double)sets : INFINITY;
+        if (table) {
+            episdet_tune_entry e;
+            std::memset(&e, 0, sizeof e);
+            e.path = c.first;
+            e.tile = c.second;
+            e.est_ms = est;
+            e.sample_ms = ms;
+            e.sample_sets = (long long)sets;
+            table->push_back(e);
+        }
+        if (est < best) {
+            best = est;
+            *path = c.first;
+            *tile = c.second;
+        }
+    }
+    return best < INFINITY ? EPISDET_OK : EPISDET_ERR_INTERNAL;
+}
+
+// The kernel AUTO runs: the tuned choice for this shape if there is one (or
+// tune now when asked), else the FP4-MFMA kernels wherever they are exact
+// (fewer than 2^24 samples per class) and the VALU kernels otherwise.
+int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune)
+{
+    const TuneKey key{order, score, ds->n, ds->W[0], ds->W[1]};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            *path = it->second.path;
+            *tile = it->second.tile;
+            return EPISDET_OK;
+        }
+    }
+    if (tune) {
+        CHK(tune_kernel(ds, order, score, nullptr, path, tile));
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[key] = TuneChoice{*path, *tile};
+        return EPISDET_OK;
+    }
+    *path = mfma_ok(ds) ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
+    *tile = 0;
+    return EPISDET_OK;
 }
 
 // Fused search: one persistent launch, then FP64 re-scoring of the
@@ -580,57 +894,20 @@ int resolve_path(const episdet_dataset *ds, const episdet_options *o)
 int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
 {
     const int order = o->order, score = o->score;
-    const int path = resolve_path(ds, o);
-    const bool mfma = path == EPISDET_PATH_FUSED_MFMA;
-    if (mfma && !mfma_ok(ds)) return EPISDET_ERR_INVALID;
-    const int tile = o->tile;
-    if (mfma && (tile < 0 || tile >= (order == 3 ? MFMA_TILES : 1))) return EPISDET_ERR_INVALID;
-    if (!mfma && order == 3 && (tile < 0 || tile >= (int)(sizeof TRI_TILES / sizeof TRI_TILES[0])))
-        return EPISDET_ERR_INVALID;
-    if (!mfma && order == 2 && (tile < 0 || tile >= (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0])))
-        return EPISDET_ERR_INVALID;
+    int path = o->path, tile = o->tile;
     const int topk = o->topk > 1 ? o->topk : 0;
     if (topk > EPISDET_TOPK_MAX) return EPISDET_ERR_INVALID;
     CHK(ensure_tables(ds, score));
     if (order == 3) CHK(ensure_pairs(ds));
+    if (path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, order, score, &path, &tile, o->autotune != 0));
     unsigned cap = o->candidate_cap > 0 ? (unsigned)o->candidate_cap : DEFAULT_CAP;
     // top-k: before the bound settles, each resident block can append about
     // 2k sets of its first units
     if (topk) cap = std::max<unsigned>(cap, (unsigned)std::min<unsigned long long>(1ull << 26, 32ull * topk * ds->cu));
     CHK(ensure_cand(ds, cap));
-    const ScoreCtx sc = make_ctx(ds, score, order);
-
-    unsigned long long *d_gstart = nullptr;
-    int2 *d_groups = nullptr;
-    TriArgs ta;
-    PairArgs pa;
-    if (order == 3) {
-        std::vector<int2> groups;
-        std::vector<unsigned long long> gstart;
-        if (mfma) {
-            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), groups, gstart);
-        } else {
-            const Tri3Tile t = TRI_TILES[tile];
-            build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
-        }
-        if (groups.empty()) return EPISDET_ERR_INVALID;
-        if (hipMalloc(&d_gstart, sizeof(unsigned long long) * gstart.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
-        if (hipMalloc(&d_groups, sizeof(int2) * groups.size()) != hipSuccess) {
-            (void)hipFree(d_gstart);
-            return EPISDET_ERR_ALLOC;
-        }
-        HIPCHK(hipMemcpy(d_gstart, gstart.data(), sizeof(unsigned long long) * gstart.size(), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_groups, groups.data(), sizeof(int2) * groups.size(), hipMemcpyHostToDevice));
-        fill_tri_common(ds, ta);
-        ta.sc = sc;
-        ta.gstart = d_gstart;
-        ta.groups = d_groups;
-        ta.ngroups = (int)groups.size();
-        ta.nunits = gstart.back();
-    } else {
-        fill_pairs(ds, pa, mfma, tile);
-        pa.sc = sc;
-    }
+    FusedPlan pl;
+    CHK(plan_fused(ds, order, score, path, tile, pl));
+    const ScoreCtx &sc = pl.sc;
 
     // One search launch; with top-k, a candidate overflow grows the buffer
     // and launches again (at most twice).
@@ -638,33 +915,8 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     int rc = EPISDET_OK;
     double kms_total = 0;
     for (int attempt = 0;; attempt++) {
-        ta.co.cand = pa.co.cand = ds->cand;
-        ta.co.cand_s32 = pa.co.cand_s32 = ds->cand_s32;
-        ta.co.cand_cap = pa.co.cand_cap = ds->cand_cap;
-        ta.co.topk = pa.co.topk = (unsigned)topk;
-        rc = init_ctl(ds);
-        if (rc != EPISDET_OK) break;
-        hipEvent_t e0, e1;
-        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-            rc = EPISDET_ERR_HIP;
-            break;
-        }
-        (void)hipEventRecord(e0, ds->stream);
-        if (topk)
-            rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_TOPK>(ds, tile, ta, ds->stream)
-                                    : dispatch_tri<MODE_TOPK>(ds, tile, ta, 0, ds->stream))
-                            : dispatch_pairs<MODE_TOPK>(ds, mfma, tile, pa, ds->stream);
-        else
-            rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream)
-                                    : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
-                            : dispatch_pairs<MODE_SEARCH>(ds, mfma, tile, pa, ds->stream);
-        (void)hipEventRecord(e1, ds->stream);
-        const bool synced = hipEventSynchronize(e1) == hipSuccess;
         float kms = 0;
-        if (synced) (void)hipEventElapsedTime(&kms, e0, e1);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc == EPISDET_OK && !synced) rc = EPISDET_ERR_HIP;
+        rc = launch_fused(ds, pl, topk, 0, pl.nunits, &kms);
         if (rc != EPISDET_OK) break;
         kms_total += kms;
         if (hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -677,8 +929,6 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
         rc = ensure_cand(ds, (unsigned)grow);
         if (rc != EPISDET_OK) break;
     }
-    if (d_gstart) (void)hipFree(d_gstart);
-    if (d_groups) (void)hipFree(d_groups);
     if (rc != EPISDET_OK) return rc;
     res->kernel_ms = kms_total;
     res->margin = sc.margin;
@@ -1025,6 +1275,28 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
     return rc;
 }
 
+int episdet_autotune(episdet_dataset *ds, int order, int score, episdet_tune_entry *entries, int capacity,
+                     int *count, int *chosen)
+{
+    if (!ds || (order != 2 && order != 3) || (score != EPISDET_SCORE_K2 && score != EPISDET_SCORE_MI) ||
+        ds->n < order || !count || !chosen || capacity < 0 || (capacity > 0 && !entries))
+        return EPISDET_ERR_INVALID;
+    std::vector<episdet_tune_entry> table;
+    int path = 0, tile = 0;
+    CHK(tune_kernel(ds, order, score, &table, &path, &tile));
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune[TuneKey{order, score, ds->n, ds->W[0], ds->W[1]}] = TuneChoice{path, tile};
+    }
+    *count = (int)table.size();
+    *chosen = -1;
+    for (int i = 0; i < (int)table.size(); i++) {
+        if (i < capacity) entries[i] = table[i];
+        if (table[i].path == path && table[i].tile == tile) *chosen = i;
+    }
+    return EPISDET_OK;
+}
+
 int episdet_run_topk(episdet_dataset *ds, const episdet_options *opt, int k, unsigned long long *packed,
                      double *scores, int *count, episdet_result *res)
 {
@@ -1084,8 +1356,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
 {
     if (!dss || !opts || !results || njobs <= 0) return EPISDET_ERR_INVALID;
     const double t0 = now_s();
-    std::vector<int> tri, pairs, solo;
-    std::vector<const episdet_dataset *> seen;   // datasets already in the graph
+    std::vector<int> tri[MFMA_TILES], pairs, solo;   // 3-way jobs by MFMA tile, 2-way jobs, the rest
+    std::vector<const episdet_dataset *> seen;     // datasets already in the graph
     for (int j = 0; j < njobs; j++) {
         episdet_dataset *ds = dss[j];
         const episdet_options &o = opts[j];
@@ -1105,40 +1377,50 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         if (o.order == 3) CHK(ensure_pairs(ds));
         CHK(ensure_cand(ds, o.candidate_cap > 0 ? (unsigned)o.candidate_cap : DEFAULT_CAP));
         results[j].prep_ms = (now_s() - tp) * 1e3;
-        // 3-way jobs go to the batched FP4 MFMA kernel, 2-way jobs to the
-        // batched FP4-MFMA pair kernel; anything else runs on its own afterwards.
+        // 3-way jobs go to the batched FP4-MFMA kernel of their tile, 2-way
+        // jobs to the batched FP4-MFMA pair kernel; anything else runs on its
+        // own afterwards.
         // A dataset's control block and candidate buffer hold one search at a
         // time, so a second job on the same dataset runs after the graph.
         const bool dup = std::find(seen.begin(), seen.end(), ds) != seen.end();
-        if (dup || o.path == EPISDET_PATH_PLAIN || !mfma_ok(ds)) {
+        int kpath = o.path, ktile = o.tile;
+        if (o.path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, o.order, o.score, &kpath, &ktile, o.autotune != 0));
+        const bool batchable =
+            kpath == EPISDET_PATH_FUSED_MFMA && mfma_ok(ds) && (ktile == 0 || (o.order == 3 && ktile == 1));
+        if (dup || !batchable) {
             solo.push_back(j);
             continue;
         }
         seen.push_back(ds);
-        if (o.order == 3) tri.push_back(j);
+        if (o.order == 3) tri[ktile].push_back(j);
         else pairs.push_back(j);
     }
     DevList dl;
     int cu = dss[0]->cu;
     // per-job kernel arguments
-    std::vector<TriArgs> ta(tri.size());
-    std::vector<unsigned long long> tstart(1, 0ull);
-    for (size_t k = 0; k < tri.size(); k++) {
-        episdet_dataset *ds = dss[tri[k]];
-        std::vector<int2> groups;
-        std::vector<unsigned long long> gstart;
-        build_groups(ds->n, M2_TY, M2_TZ, groups, gstart);
-        unsigned long long *d_gstart;
-        int2 *d_groups;
-        CHK(dl.put(&d_gstart, gstart.data(), gstart.size()));
-        CHK(dl.put(&d_groups, groups.data(), groups.size()));
-        fill_tri_common(ds, ta[k]);
-        ta[k].sc = make_ctx(ds, opts[tri[k]].score, 3);
-        ta[k].gstart = d_gstart;
-        ta[k].groups = d_groups;
-        ta[k].ngroups = (int)groups.size();
-        ta[k].nunits = gstart.back();
-        tstart.push_back(tstart.back() + gstart.back());
+    std::vector<TriArgs> ta[MFMA_TILES];
+    std::vector<unsigned long long> tstart[MFMA_TILES];
+    for (int tl = 0; tl < MFMA_TILES; tl++) {
+        ta[tl].resize(tri[tl].size());
+        tstart[tl].assign(1, 0ull);
+        for (size_t k = 0; k < tri[tl].size(); k++) {
+            episdet_dataset *ds = dss[tri[tl][k]];
+            std::vector<int2> groups;
+            std::vector<unsigned long long> gstart;
+            build_groups(ds->n, mfma_ty(tl), mfma_tz(tl), groups, gstart);
+            unsigned long long *d_gstart;
+            int2 *d_groups;
+            CHK(dl.put(&d_gstart, gstart.data(), gstart.size()));
+            CHK(dl.put(&d_groups, groups.data(), groups.size()));
+            TriArgs &a = ta[tl][k];
+            fill_tri_common(ds, a);
+            a.sc = make_ctx(ds, opts[tri[tl][k]].score, 3);
+            a.gstart = d_gstart;
+            a.groups = d_groups;
+            a.ngroups = (int)groups.size();
+            a.nunits = gstart.back();
+            tstart[tl].push_back(tstart[tl].back() + gstart.back());
+        }
     }
     std::vector<PairArgs> pa(pairs.size());
     std::vector<unsigned long long> pstart(1, 0ull);
@@ -1148,7 +1430,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         pa[k].sc = make_ctx(ds, opts[pairs[k]].score, 2);
         pstart.push_back(pstart.back() + pa[k].nunits);
     }
-    std::vector<int> batched(tri);
+    std::vector<int> batched;
+    for (int tl = 0; tl < MFMA_TILES; tl++) batched.insert(batched.end(), tri[tl].begin(), tri[tl].end());
     batched.insert(batched.end(), pairs.begin(), pairs.end());
     std::vector<RescoreJob> rj(batched.size());
     std::vector<unsigned long long *> qp(batched.size());
@@ -1177,18 +1460,21 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         bp[k] = &ds->ctl->best32;
         cp[k] = &ds->ctl->cand_count;
     }
-    TriArgs *d_ta = nullptr;
+    TriArgs *d_ta[MFMA_TILES] = {};
     PairArgs *d_pa = nullptr;
-    unsigned long long *d_tstart = nullptr, *d_pstart = nullptr, *d_bq = nullptr;
+    unsigned long long *d_tstart[MFMA_TILES] = {}, *d_pstart = nullptr, *d_bq = nullptr;
     RescoreJob *d_rj = nullptr;
     unsigned long long **d_qp = nullptr;
     unsigned **d_bp = nullptr, **d_cp = nullptr;
-    const unsigned long long zero2[2] = {0ull, 0ull};
-    CHK(dl.put(&d_ta, ta.data(), ta.size()));
+    constexpr int NQ = MFMA_TILES + 1;   // one work queue per batched kernel
+    const unsigned long long zeros[NQ] = {};
+    for (int tl = 0; tl < MFMA_TILES; tl++) {
+        CHK(dl.put(&d_ta[tl], ta[tl].data(), ta[tl].size()));
+        CHK(dl.put(&d_tstart[tl], tstart[tl].data(), tstart[tl].size()));
+    }
     CHK(dl.put(&d_pa, pa.data(), pa.size()));
-    CHK(dl.put(&d_tstart, tstart.data(), tstart.size()));
     CHK(dl.put(&d_pstart, pstart.data(), pstart.size()));
-    CHK(dl.put(&d_bq, zero2, 2));
+    CHK(dl.put(&d_bq, zeros, NQ));
     CHK(dl.put(&d_rj, rj.data(), rj.size()));
     CHK(dl.put(&d_qp, qp.data(), qp.size()));
     CHK(dl.put(&d_bp, bp.data(), bp.size()));
@@ -1196,40 +1482,36 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
 
     double gms = 0;
     if (!batched.empty()) {
-        int per3 = 0, per2 = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per3, (const void *)k_triplets_mfma2<MODE_SEARCH>, 512, 0) !=
-                hipSuccess ||
-            per3 < 1)
-            per3 = 1;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per2, (const void *)k_pairs_mfma<MODE_SEARCH>, 512, 0) !=
-                hipSuccess ||
-            per2 < 1)
-            per2 = 1;
-        hipStream_t st, st2;
+        const int blocks2 = occupancy_blocks((const void *)k_triplets_mfma2<MODE_SEARCH>, 512, cu);
+        const int blocks1 = occupancy_blocks((const void *)k_triplets_mfma<MODE_SEARCH>, 512, cu);
+        const int blocksp = occupancy_blocks((const void *)k_pairs_mfma<MODE_SEARCH>, 512, cu);
+        hipStream_t st, br[NQ];
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
-        hipEvent_t fork, join;
+        for (int q = 0; q < NQ; q++) HIPCHK(hipStreamCreateWithFlags(&br[q], hipStreamNonBlocking));
+        hipEvent_t fork, join[NQ];
         HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        for (int q = 0; q < NQ; q++) HIPCHK(hipEventCreateWithFlags(&join[q], hipEventDisableTiming));
         hipGraph_t graph;
         hipGraphExec_t exec;
-        // one graph: reset, then the 3-way and the 2-way batch as parallel
-        // branches (the second fills CUs the first's tail leaves idle), then
-        // FP64 re-scoring
+        // one graph: reset, then one branch per batched kernel (3-way tile 0,
+        // 3-way tile 1, 2-way) running side by side, so each fills the CUs
+        // the others' tails leave idle, then FP64 re-scoring
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const int nb = (int)batched.size();
-        hipLaunchKernelGGL(k_init_many, dim3((std::max(nb, 2) + 63) / 64), dim3(64), 0, st, d_qp, d_bp, d_cp, nb, d_bq,
-                           2);
+        hipLaunchKernelGGL(k_init_many, dim3((std::max(nb, NQ) + 63) / 64), dim3(64), 0, st, d_qp, d_bp, d_cp, nb,
+                           d_bq, NQ);
         HIPCHK(hipEventRecord(fork, st));
-        HIPCHK(hipStreamWaitEvent(st2, fork, 0));
-        if (!tri.empty()) {
+        for (int q = 0; q < NQ; q++) HIPCHK(hipStreamWaitEvent(br[q], fork, 0));
+        for (int tl = 0; tl < MFMA_TILES; tl++) {
+            if (tri[tl].empty()) continue;
             BatchTri bt;
-            bt.jobs = d_ta;
-            bt.job_start = d_tstart;
-            bt.njobs = (int)tri.size();
-            bt.total = tstart.back();
-            bt.queue = d_bq;
-            hipLaunchKernelGGL((k_triplets_mfma2<MODE_SEARCH>), dim3(per3 * cu), dim3(512), 0, st, bt);
+            bt.jobs = d_ta[tl];
+            bt.job_start = d_tstart[tl];
+            bt.njobs = (int)tri[tl].size();
+            bt.total = tstart[tl].back();
+            bt.queue = d_bq + tl;
+            if (tl == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE_SEARCH>), dim3(blocks1), dim3(512), 0, br[tl], bt);
+            else hipLaunchKernelGGL((k_triplets_mfma2<MODE_SEARCH>), dim3(blocks2), dim3(512), 0, br[tl], bt);
         }
         if (!pairs.empty()) {
             BatchPair bpr;
@@ -1237,11 +1519,13 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             bpr.job_start = d_pstart;
             bpr.njobs = (int)pairs.size();
             bpr.total = pstart.back();
-            bpr.queue = d_bq + 1;
-            hipLaunchKernelGGL((k_pairs_mfma<MODE_SEARCH>), dim3(per2 * cu), dim3(512), 0, st2, bpr);
+            bpr.queue = d_bq + MFMA_TILES;
+            hipLaunchKernelGGL((k_pairs_mfma<MODE_SEARCH>), dim3(blocksp), dim3(512), 0, br[MFMA_TILES], bpr);
         }
-        HIPCHK(hipEventRecord(join, st2));
-        HIPCHK(hipStreamWaitEvent(st, join, 0));
+        for (int q = 0; q < NQ; q++) {
+            HIPCHK(hipEventRecord(join[q], br[q]));
+            HIPCHK(hipStreamWaitEvent(st, join[q], 0));
+        }
         hipLaunchKernelGGL(k_rescore_batch, dim3(8 * cu), dim3(64), 0, st, d_rj, nb);
         HIPCHK(hipStreamEndCapture(st, &graph));
         HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
@@ -1260,8 +1544,10 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);
         (void)hipEventDestroy(fork);
-        (void)hipEventDestroy(join);
-        (void)hipStreamDestroy(st2);
+        for (int q = 0; q < NQ; q++) {
+            (void)hipEventDestroy(join[q]);
+            (void)hipStreamDestroy(br[q]);
+        }
         (void)hipStreamDestroy(st);
     }
     for (size_t k = 0; k < batched.size(); k++) {
@@ -1277,7 +1563,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         CHK(collect_best(ds, opts[j].order, opts[j].score, (int)hc.cand_count, &r));
         r.kernel_ms = gms;
         r.margin = make_ctx(ds, opts[j].score, opts[j].order).margin;
-        r.path = opts[j].order == 3 ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
+        r.path = EPISDET_PATH_FUSED_MFMA;
         r.rounds = episdet_reference_rounds(opts[j].order, ds->n);
         r.sets = sets_total(opts[j].order, ds->n);
     }
@@ -1285,7 +1571,6 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         const double pm = results[j].prep_ms;
         episdet_options o = opts[j];
         o.rebuild = 0;
-        if (o.path != EPISDET_PATH_PLAIN) o.path = EPISDET_PATH_AUTO;
         CHK(episdet_run(dss[j], &o, &results[j]));
         results[j].prep_ms = pm;
     }

@@ -51,7 +51,12 @@ def _load(name):
 class Options(C.Structure):
     _fields_ = [("order", C.c_int), ("score", C.c_int), ("path", C.c_int), ("tile", C.c_int),
                 ("plain_rounds", C.c_int), ("candidate_cap", C.c_int), ("rebuild", C.c_int),
-                ("topk", C.c_int), ("reserved", C.c_int * 6)]
+                ("topk", C.c_int), ("autotune", C.c_int), ("reserved", C.c_int * 5)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("path", C.c_int), ("tile", C.c_int), ("est_ms", C.c_double), ("sample_ms", C.c_double),
+                ("sample_sets", C.c_longlong)]
 
 
 class Result(C.Structure):
@@ -89,7 +94,7 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 # Every symbol include/episdet.h and include/episdet_data.h declare.
 SEARCH_SYMBOLS = ("episdet_search", "episdet_dataset_create", "episdet_dataset_destroy", "episdet_run",
-                  "episdet_run_topk", "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
+                  "episdet_run_topk", "episdet_autotune", "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
                   "episdet_checksum", "episdet_n_choose_k", "episdet_reference_rounds", "episdet_status_string",
                   "episdet_device_info")
 DATA_SYMBOLS = ("episdet_gen_defaults", "episdet_generate", "episdet_padded_words", "episdet_write_dataset",
@@ -137,6 +142,9 @@ def search_lib():
         lib.episdet_run_topk.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, C.POINTER(C.c_ulonglong),
                                          C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(Result)]
         lib.episdet_run_topk.restype = C.c_int
+        lib.episdet_autotune.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(TuneEntry), C.c_int,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        lib.episdet_autotune.restype = C.c_int
         lib.episdet_run_batch.argtypes = [C.POINTER(C.c_void_p), C.POINTER(Options), C.c_int, C.POINTER(Result),
                                           C.POINTER(C.c_double)]
         lib.episdet_run_batch.restype = C.c_int
@@ -265,14 +273,18 @@ def search(order, score, cases, controls, num_cases, num_controls):
     return packed.value, best.value, rounds.value, secs.value
 
 
-def run_batch(datasets, orders, scores, rebuild=False, path=PATH_AUTO):
-    """Many resident searches in one graph launch; returns (results, batch_ms)."""
+def run_batch(datasets, orders, scores, rebuild=False, path=PATH_AUTO, autotune=False, paths=None, tiles=None):
+    """Many resident searches in one graph launch; returns (results, batch_ms).
+    paths / tiles: optional per-job path and tile (default: `path`, tile 0)."""
     n = len(datasets)
     hs = (C.c_void_p * n)(*[d._h for d in datasets])
     opts = (Options * n)()
     for j in range(n):
-        opts[j].order, opts[j].score, opts[j].path = orders[j], scores[j], path
+        opts[j].order, opts[j].score = orders[j], scores[j]
+        opts[j].path = paths[j] if paths is not None else path
+        opts[j].tile = tiles[j] if tiles is not None else 0
         opts[j].rebuild = 1 if rebuild else 0
+        opts[j].autotune = 1 if autotune else 0
     res = (Result * n)()
     ms = C.c_double()
     rc = search_lib().episdet_run_batch(hs, opts, n, res, C.byref(ms))
@@ -344,14 +356,28 @@ class Dataset:
         except Exception:
             pass
 
-    def run(self, order, score=K2, path=PATH_AUTO, tile=0, plain_rounds=0, candidate_cap=0, rebuild=False) -> dict:
+    def run(self, order, score=K2, path=PATH_AUTO, tile=0, plain_rounds=0, candidate_cap=0, rebuild=False,
+            autotune=False) -> dict:
         o = Options(order=order, score=score, path=path, tile=tile, plain_rounds=plain_rounds,
-                    candidate_cap=candidate_cap, rebuild=1 if rebuild else 0)
+                    candidate_cap=candidate_cap, rebuild=1 if rebuild else 0, autotune=1 if autotune else 0)
         r = Result()
         rc = self._lib.episdet_run(self._h, C.byref(o), C.byref(r))
         if rc:
             raise EpisdetError(rc, "episdet_run")
         return r.as_dict()
+
+    def autotune(self, order, score=K2):
+        """Times every kernel configuration for this shape and caches the
+        fastest for AUTO: (list of dicts, index of the chosen one)."""
+        cap = 8
+        arr = (TuneEntry * cap)()
+        cnt, chosen = C.c_int(), C.c_int()
+        rc = self._lib.episdet_autotune(self._h, order, score, arr, cap, C.byref(cnt), C.byref(chosen))
+        if rc:
+            raise EpisdetError(rc, "episdet_autotune")
+        out = [{"path": e.path, "tile": e.tile, "est_ms": e.est_ms, "sample_ms": e.sample_ms,
+                "sample_sets": e.sample_sets} for e in arr[:min(cnt.value, cap)]]
+        return out, chosen.value
 
     def topk(self, order, k, score=K2, path=PATH_AUTO, tile=0, rebuild=False):
         """The k best sets by FP64 score (best first): (packed list, scores

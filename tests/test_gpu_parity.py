@@ -267,6 +267,12 @@ def test_batch_of_uneven_jobs_matches_oracle():
     for rep in range(2):
         res2, _ = episdet.run_batch(dss, [o for o, _, _ in jobs], [s for _, s, _ in jobs], rebuild=rep == 1)
         assert [r["packed"] for r in res2] == [r["packed"] for r in res]
+    say("explicit MFMA tiles 0 and 1 side by side in one graph")
+    M = episdet.PATH_FUSED_MFMA
+    res4, _ = episdet.run_batch(dss, [o for o, _, _ in jobs], [s for _, s, _ in jobs], paths=[M] * len(jobs),
+                                tiles=[(j // 2) % 2 if o == 3 else 0 for j, (o, _, _) in enumerate(jobs)])
+    assert [r["packed"] for r in res4] == [r["packed"] for r in res]
+    assert [r["best_score"] for r in res4] == [r["best_score"] for r in res]
     say("one dataset in two jobs of one batch")
     res3, _ = episdet.run_batch([dss[1], dss[1], dss[0]], [3, 2, 2], [episdet.K2, episdet.MI, episdet.K2])
     assert res3[0]["packed"] == res[1]["packed"] and res3[2]["packed"] == res[0]["packed"]
@@ -318,3 +324,24 @@ def test_topk_more_than_all_sets_and_errors():
             ds.topk(3, 10, path=episdet.PATH_PLAIN)
         with pytest.raises(episdet.EpisdetError):
             ds.topk(3, episdet.TOPK_MAX + 1)
+
+
+# ---------------------------------------------------------- autotuner ----
+
+@pytest.mark.parametrize("order,n,Nc,Nt", [(3, 150, 3000, 2500), (2, 700, 5000, 4000)])
+def test_autotune_then_auto_matches_oracle(order, n, Nc, Nt):
+    """Every candidate configuration is timed on a window of its work units,
+    the fastest is cached, and AUTO (single and batched) then runs it with
+    the oracle's winner."""
+    syn = episdet.generate(n, Nc, Nt, seed=50 + order, planted_sets=1, planted_order=order)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        say(f"autotune order={order} n={n}")
+        table, chosen = ds.autotune(order, episdet.K2)
+        expect = (5 if order == 3 else 3)
+        assert len(table) == expect and 0 <= chosen < expect
+        assert all(e["sample_ms"] > 0 and e["sample_sets"] > 0 and e["est_ms"] > 0 for e in table)
+        assert table[chosen]["est_ms"] == min(e["est_ms"] for e in table)
+        r = check_best(ds, syn.cases, syn.controls, Nc, Nt, order, episdet.K2, path=episdet.PATH_AUTO)
+        assert (r["path"], r["tile"]) == (table[chosen]["path"], table[chosen]["tile"])
+        res, _ = episdet.run_batch([ds], [order], [episdet.K2])
+        assert res[0]["packed"] == r["packed"] and res[0]["best_score"] == r["best_score"]
