@@ -350,6 +350,60 @@ __device__ __forceinline__ void m2_issue(const uint32_t *base, int W, int X, int
     }
 }
 
+// Buffer-resource form of the same loads: the row offsets of a work unit are
+// computed once (m2_row_offsets) and every chunk only adds its word offset
+// as the scalar soffset, so a chunk's loads cost no vector address
+// arithmetic.  Reads at or past num_records (the class's plane bytes)
+// return zero.
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+__device__ void m2_buffer_load_lds(i32x4_t rsrc, __attribute__((address_space(3))) uint32_t *lds, int size,
+                                   int voffset, int soffset, int offset, int aux) __asm("llvm.amdgcn.raw.buffer.load.lds");
+
+constexpr int M2_LOADS_PER_WAVE = (M2_LOADS + 3) / 4;
+
+__device__ __forceinline__ i32x4_t m2_rsrc(const uint32_t *base, unsigned bytes)
+{
+    // base and bytes are the same for a whole wave (one class per wave);
+    // readfirstlane puts the descriptor in scalar registers
+    const unsigned long long a = (unsigned long long)base;
+    i32x4_t r;
+    r.x = __builtin_amdgcn_readfirstlane((int)(unsigned)a);
+    r.y = __builtin_amdgcn_readfirstlane((int)(unsigned)(a >> 32));   // stride 0: raw byte-addressed
+    r.z = __builtin_amdgcn_readfirstlane((int)bytes);
+    r.w = 0x00020000;                                                  // 32-bit data format
+    return r;
+}
+
+// Byte offset, within the class's planes, of slot q of each LDS row this
+// lane loads (load k of wave wih fills rows 8(wih + 4k) .. +7).
+__device__ __forceinline__ void m2_row_offsets(int W, int X, int y0, int z0, int wih, int lane,
+                                               unsigned (&roff)[M2_LOADS_PER_WAVE])
+{
+    const int rl = lane >> 3, q = (lane & 7) ^ rl;
+#pragma unroll
+    for (int k = 0; k < M2_LOADS_PER_WAVE; k++) {
+        const int r = (wih + 4 * k) * 8 + rl;
+        int row;
+        if (r < M2_YROW) row = 2 * X + r;
+        else if (r < M2_ZROW) row = 2 * y0 + (r - M2_YROW);
+        else if (r < M2_ROWS) row = 2 * z0 + (r - M2_ZROW);
+        else row = 2 * X;                     // padding rows are never read
+        roff[k] = ((unsigned)row * (unsigned)W + (unsigned)(q * 4)) * 4u;
+    }
+}
+
+__device__ __forceinline__ void m2_issue_buf(i32x4_t rsrc, const unsigned (&roff)[M2_LOADS_PER_WAVE], int w0,
+                                             uint32_t *buf, int wih)
+{
+#pragma unroll
+    for (int k = 0; k < M2_LOADS_PER_WAVE; k++) {
+        const int ii = wih + 4 * k;
+        if (ii < M2_LOADS)
+            m2_buffer_load_lds(rsrc, (__attribute__((address_space(3))) uint32_t *)(buf + ii * 8 * M2_KW), 16,
+                               (int)roff[k], w0 * 4, 0, 0);
+    }
+}
+
 // Word offset of logical 16-byte slot q of LDS row r.
 __device__ __forceinline__ int m2_at(int r, int q) { return r * M2_KW + ((q ^ (r & 7)) << 2); }
 
@@ -432,6 +486,9 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
         const int X = (int)(u - p.gstart[g]);
         const int2 gt = p.groups[g];
         const int y0 = gt.x * M2_TY, z0 = gt.y * M2_TZ;
+        const i32x4_t rsrc = m2_rsrc(base, cls ? p.bytes1 : p.bytes0);
+        unsigned roff[M2_LOADS_PER_WAVE];
+        m2_row_offsets(W, X, y0, z0, wy, lane, roff);
 
         v16f_t acc[2][4];
 #pragma unroll
@@ -441,12 +498,12 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
 
-        if (nch_mine > 0) m2_issue(base, W, X, y0, z0, 0, buf0, wy, lane);
+        if (nch_mine > 0) m2_issue_buf(rsrc, roff, 0, buf0, wy);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int c = 0; c < nch; c++) {
             uint32_t *cur = (c & 1) ? buf1 : buf0;
-            if (c + 1 < nch_mine) m2_issue(base, W, X, y0, z0, (c + 1) * M2_KW, (c & 1) ? buf0 : buf1, wy, lane);
+            if (c + 1 < nch_mine) m2_issue_buf(rsrc, roff, (c + 1) * M2_KW, (c & 1) ? buf0 : buf1, wy);
             if (c < nch_mine) {
 #pragma unroll
                 for (int s = 0; s < M2_KW / 8; s++) {

@@ -363,6 +363,7 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
 
 struct TriArgs {
     const uint32_t *p0, *p1;
+    unsigned bytes0, bytes1;             // size of each class's planes (below 4 GiB on MFMA paths)
     int W0, W1, n, N0, N1;
     const uint4 *singles;
     const uint4 *pairs;                  // [n][n][2] uint4, a < b

@@ -309,7 +309,16 @@ int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
 
 // The MFMA path accumulates counts in FP32: exact while every class has
 // fewer than 2^24 samples.
-bool mfma_ok(const episdet_dataset *ds) { return ds->N[0] < (1 << 24) && ds->N[1] < (1 << 24); }
+size_t plane_bytes(const episdet_dataset *ds, int c) { return (size_t)ds->nrows * 2 * ds->W[c] * sizeof(uint32_t); }
+
+// The MFMA kernels accumulate counts in FP32, exact while every class has
+// fewer than 2^24 samples, and address each class's planes with 32-bit byte
+// offsets.
+bool mfma_ok(const episdet_dataset *ds)
+{
+    return ds->N[0] < (1 << 24) && ds->N[1] < (1 << 24) && plane_bytes(ds, 0) < (1ull << 32) &&
+           plane_bytes(ds, 1) < (1ull << 32);
+}
 
 template <int MODE>
 int launch_pairs_batch(int R, int cu, const BatchPair &bp, hipStream_t s)
@@ -383,6 +392,8 @@ void fill_tri_common(episdet_dataset *ds, TriArgs &a)
     std::memset(&a, 0, sizeof a);
     a.p0 = ds->planes[0];
     a.p1 = ds->planes[1];
+    a.bytes0 = (unsigned)std::min<size_t>(plane_bytes(ds, 0), 0xFFFFFFFFu);
+    a.bytes1 = (unsigned)std::min<size_t>(plane_bytes(ds, 1), 0xFFFFFFFFu);
     a.W0 = ds->W[0];
     a.W1 = ds->W[1];
     a.n = ds->n;
