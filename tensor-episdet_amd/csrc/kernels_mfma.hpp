@@ -295,7 +295,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
 // Same contraction and FP4 encoding, restructured so the chunk pipeline costs
 // no VALU work and one barrier:
 //  * the X, Y and Z rows of a 1024-sample chunk go straight from global
-//    memory to LDS (global_load_lds_dwordx4), into one of two buffers while
+//    memory to LDS (buffer_load_dwordx4 ... lds), into one of two buffers while
 //    the other is contracted;
 //  * LDS rows are 128 bytes without padding; the eight 16-byte slots of row r
 //    are XOR-swizzled by r & 7, so the lanes of a wave, which read up to eight
@@ -319,42 +319,14 @@ constexpr int M2_DUMP_WORDS = 2 * 4 * 4 * 16 * 64;          // class x wave x ti
 constexpr int M2_SMEM_WORDS = M2_STAGE_WORDS > M2_DUMP_WORDS ? M2_STAGE_WORDS : M2_DUMP_WORDS;
 constexpr int M2_YROW = 2, M2_ZROW = 2 + 2 * M2_TY;
 
-typedef __attribute__((address_space(3))) void lds_void_t;
 
-// Global row of LDS row r (X planes, then Y rows 2y+j, then Z rows 2z+k).
-__device__ __forceinline__ const uint32_t *m2_rowptr(const uint32_t *base, int W, int X, int y0, int z0, int r)
-{
-    if (r < M2_YROW) return base + ((size_t)X * 2 + r) * W;
-    if (r < M2_ZROW) {
-        const int rr = r - M2_YROW;
-        return base + ((size_t)(y0 + (rr >> 1)) * 2 + (rr & 1)) * W;
-    }
-    if (r < M2_ROWS) {
-        const int rr = r - M2_ZROW;
-        return base + ((size_t)(z0 + (rr >> 1)) * 2 + (rr & 1)) * W;
-    }
-    return base + (size_t)X * 2 * W;     // padding rows are never read
-}
-
-// This wave's share of the direct-to-LDS loads of chunk w0 into buf.  One
-// instruction fills eight rows: lane L writes slot L of a 1 KB block, i.e.
-// row 8*ii + L/8, physical slot L%8, which holds logical slot (L%8) ^ (L/8).
-__device__ __forceinline__ void m2_issue(const uint32_t *base, int W, int X, int y0, int z0, int w0, uint32_t *buf,
-                                         int wih, int lane)
-{
-    const int rl = lane >> 3, q = (lane & 7) ^ rl;
-#pragma unroll
-    for (int ii = wih; ii < M2_LOADS; ii += 4) {
-        const uint32_t *src = m2_rowptr(base, W, X, y0, z0, ii * 8 + rl) + w0 + q * 4;
-        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)(buf + ii * 8 * M2_KW), 16, 0, 0);
-    }
-}
-
-// Buffer-resource form of the same loads: the row offsets of a work unit are
+// Chunk loads, straight from global memory to LDS (buffer_load ... lds)
+// through a scalar buffer descriptor: the row offsets of a work unit are
 // computed once (m2_row_offsets) and every chunk only adds its word offset
 // as the scalar soffset, so a chunk's loads cost no vector address
-// arithmetic.  Reads at or past num_records (the class's plane bytes)
-// return zero.
+// arithmetic.  One instruction fills eight 128-byte LDS rows: lane L writes
+// physical slot L%8 of row L/8, which holds logical slot (L%8) ^ (L/8).
+// Reads at or past num_records (the class's plane bytes) return zero.
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 __device__ void m2_buffer_load_lds(i32x4_t rsrc, __attribute__((address_space(3))) uint32_t *lds, int size,
                                    int voffset, int soffset, int offset, int aux) __asm("llvm.amdgcn.raw.buffer.load.lds");

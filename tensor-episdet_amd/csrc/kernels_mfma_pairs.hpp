@@ -18,7 +18,7 @@
 // SNPs x 2 planes) x all 128 B rows = 2 x 4 MFMA tiles, 128 accumulators, so
 // each A operand feeds four MFMAs and each B operand two.  The rows of a
 // 512-sample chunk (256 A rows + 128 B rows, 64 bytes each) go straight from
-// global memory to LDS (global_load_lds_dwordx4), double-buffered; the four
+// global memory to LDS (buffer_load_dwordx4 ... lds), double-buffered; the four
 // 16-byte slots of row r are XOR-swizzled by (r >> 1) & 3, so the eight
 // consecutive rows a group of lanes reads at one logical slot cover all
 // banks once.  After the last chunk the accumulators go to LDS in two
@@ -66,25 +66,33 @@ __device__ __forceinline__ void p2_unrank(unsigned long long u, int &ta, int &tb
     }
 }
 
-// Global row of LDS row r: A rows 2a+i, then B rows 2b+j.
-__device__ __forceinline__ const uint32_t *p2_rowptr(const uint32_t *base, int W, int a0, int b0, int r)
-{
-    if (r < P2_AROWS) return base + ((size_t)a0 * 2 + r) * W;
-    return base + ((size_t)b0 * 2 + (r - P2_AROWS)) * W;
-}
+// Chunk loads through a scalar buffer descriptor (see m2_row_offsets): the
+// row offsets of a unit are computed once, each chunk adds its word offset
+// as the scalar soffset.  One instruction fills sixteen 64-byte LDS rows:
+// lane L writes physical slot L%4 of row L/4, which holds logical slot
+// (L%4) ^ ((L/4 >> 1) & 3).
+constexpr int P2_LOADS_PER_WAVE = P2_LOADS / 4;
 
-// This wave's share of the direct-to-LDS loads of chunk w0 into buf.  One
-// instruction fills sixteen rows: lane L writes physical slot L%4 of row
-// L/4 of the block, which holds logical slot (L%4) ^ ((L/4 >> 1) & 3).
-__device__ __forceinline__ void p2_issue(const uint32_t *base, int W, int a0, int b0, int w0, uint32_t *buf,
-                                         int wih, int lane)
+__device__ __forceinline__ void p2_row_offsets(int W, int a0, int b0, int wih, int lane,
+                                               unsigned (&roff)[P2_LOADS_PER_WAVE])
 {
     const int rl = lane >> 2, q = (lane & 3) ^ ((rl >> 1) & 3);
 #pragma unroll
-    for (int ii = wih; ii < P2_LOADS; ii += 4) {
-        const uint32_t *src = p2_rowptr(base, W, a0, b0, ii * P2_ROWS_PER_LOAD + rl) + w0 + q * 4;
-        __builtin_amdgcn_global_load_lds((const void *)src, (lds_void_t *)(buf + ii * P2_ROWS_PER_LOAD * P2_KW), 16,
-                                         0, 0);
+    for (int k = 0; k < P2_LOADS_PER_WAVE; k++) {
+        const int r = (wih + 4 * k) * P2_ROWS_PER_LOAD + rl;
+        const int row = r < P2_AROWS ? 2 * a0 + r : 2 * b0 + (r - P2_AROWS);
+        roff[k] = ((unsigned)row * (unsigned)W + (unsigned)(q * 4)) * 4u;
+    }
+}
+
+__device__ __forceinline__ void p2_issue_buf(i32x4_t rsrc, const unsigned (&roff)[P2_LOADS_PER_WAVE], int w0,
+                                             uint32_t *buf, int wih)
+{
+#pragma unroll
+    for (int k = 0; k < P2_LOADS_PER_WAVE; k++) {
+        const int ii = wih + 4 * k;
+        m2_buffer_load_lds(rsrc, (__attribute__((address_space(3))) uint32_t *)(buf + ii * P2_ROWS_PER_LOAD * P2_KW),
+                           16, (int)roff[k], w0 * 4, 0, 0);
     }
 }
 
@@ -140,6 +148,9 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
         const int W = cls ? p.W1 : p.W0;
         const int nch_mine = W / P2_KW;
         const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / P2_KW;
+        const i32x4_t rsrc = m2_rsrc(base, cls ? p.bytes1 : p.bytes0);
+        unsigned roff[P2_LOADS_PER_WAVE];
+        p2_row_offsets(W, a0, b0, wy, lane, roff);
 
         v16f_t acc[2][4];
 #pragma unroll
@@ -149,12 +160,12 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
 
-        if (nch_mine > 0) p2_issue(base, W, a0, b0, 0, buf0, wy, lane);
+        if (nch_mine > 0) p2_issue_buf(rsrc, roff, 0, buf0, wy);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int c = 0; c < nch; c++) {
             uint32_t *cur = (c & 1) ? buf1 : buf0;
-            if (c + 1 < nch_mine) p2_issue(base, W, a0, b0, (c + 1) * P2_KW, (c & 1) ? buf0 : buf1, wy, lane);
+            if (c + 1 < nch_mine) p2_issue_buf(rsrc, roff, (c + 1) * P2_KW, (c & 1) ? buf0 : buf1, wy);
             if (c < nch_mine) {
 #pragma unroll
                 for (int s = 0; s < P2_KW / 8; s++) {

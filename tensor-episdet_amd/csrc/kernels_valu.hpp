@@ -196,6 +196,7 @@ __device__ __forceinline__ void publish_candidates(const CandOut &co, float marg
 
 struct PairArgs {
     const uint32_t *p0, *p1;     // cases / controls planes
+    unsigned bytes0, bytes1;     // size of each class's planes (below 4 GiB on MFMA paths)
     int W0, W1, n, N0, N1;
     const uint4 *singles;
     ScoreCtx sc;

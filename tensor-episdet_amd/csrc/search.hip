@@ -415,6 +415,8 @@ void fill_pair_common(episdet_dataset *ds, PairArgs &a, int T)
     std::memset(&a, 0, sizeof a);
     a.p0 = ds->planes[0];
     a.p1 = ds->planes[1];
+    a.bytes0 = (unsigned)std::min<size_t>(plane_bytes(ds, 0), 0xFFFFFFFFu);
+    a.bytes1 = (unsigned)std::min<size_t>(plane_bytes(ds, 1), 0xFFFFFFFFu);
     a.W0 = ds->W[0];
     a.W1 = ds->W[1];
     a.n = ds->n;
