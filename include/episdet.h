@@ -205,12 +205,13 @@ int episdet_tables(episdet_dataset *ds, int order, const int *sets, int m, unsig
 
 /* Every set's table from the fused kernel itself (small n only):
  * out[rank][2][3^order] in colex rank order, rank = C(Z,3)+C(Y,2)+X or
- * C(B,2)+A; capacity in elements.  path: FUSED_VALU or FUSED_MFMA. */
-int episdet_dump_tables(episdet_dataset *ds, int order, int path, unsigned *out, size_t capacity);
+ * C(B,2)+A; capacity in elements.  path: FUSED_VALU or FUSED_MFMA; tile: the
+ * kernel tile id of that path (as in episdet_options). */
+int episdet_dump_tables(episdet_dataset *ds, int order, int path, int tile, unsigned *out, size_t capacity);
 
 /* Order-independent 64-bit checksum over every set's table computed by the
  * fused kernel (sum over sets of a hash of rank and the 2 x 3^k cells). */
-int episdet_checksum(episdet_dataset *ds, int order, int path, unsigned long long *checksum);
+int episdet_checksum(episdet_dataset *ds, int order, int path, int tile, unsigned long long *checksum);
 
 /* Exact binomial C(n,k) (helper.cu:758-768). */
 unsigned long long episdet_n_choose_k(unsigned n, unsigned k);

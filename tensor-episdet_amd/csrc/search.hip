@@ -24,6 +24,7 @@
 #include "episdet.h"
 #include "kernels_mfma.hpp"
 #include "kernels_mfma_pairs.hpp"
+#include "kernels_mfma3.hpp"
 
 using namespace episdet;
 
@@ -269,11 +270,27 @@ int dispatch_tri(episdet_dataset *ds, int tile, TriArgs &a, int grid_override, h
     return EPISDET_ERR_INVALID;
 }
 
-// MFMA tiles: 0 = version 2 (64 x 64 units, direct-to-LDS double buffering),
-// 1 = version 1 (32 x 64 units, register-staged chunks).
-constexpr int MFMA_TILES = 2;
-inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : M2_TY; }
-inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : M2_TZ; }
+// MFMA tiles: 0 = version 2 (64 x 64 units, one 512-thread workgroup per
+// CU), 1 = version 1 (32 x 64 units, register-staged chunks), 2 = version 3
+// (64 x 32 units, two 256-thread workgroups per CU).
+constexpr int MFMA_TILES = 3;
+inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : (tile == 2 ? M3_TY : M2_TY); }
+inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : (tile == 2 ? M3_TZ : M2_TZ); }
+inline int mfma_threads(int tile) { return tile == 2 ? 256 : 512; }
+template <int MODE>
+const void *mfma_kernel(int tile)
+{
+    if (tile == 1) return (const void *)k_triplets_mfma<MODE>;
+    if (tile == 2) return (const void *)k_triplets_mfma3<MODE>;
+    return (const void *)k_triplets_mfma2<MODE>;
+}
+template <int MODE>
+void launch_tri_mfma(int tile, int blocks, const BatchTri &bt, hipStream_t s)
+{
+    if (tile == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(blocks), dim3(512), 0, s, bt);
+    else if (tile == 2) hipLaunchKernelGGL((k_triplets_mfma3<MODE>), dim3(blocks), dim3(256), 0, s, bt);
+    else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(blocks), dim3(512), 0, s, bt);
+}
 
 static_assert(sizeof(TriArgs) <= 1024 && sizeof(PairArgs) <= 1024, "argument slot too small");
 
@@ -291,9 +308,10 @@ int upload_single(episdet_dataset *ds, const A &a, unsigned long long units)
 template <int MODE>
 int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
 {
-    const void *k = tile == 1 ? (const void *)k_triplets_mfma<MODE> : (const void *)k_triplets_mfma2<MODE>;
+    const void *k = mfma_kernel<MODE>(tile);
     int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, 512, 0) != hipSuccess || per < 1) per = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, mfma_threads(tile), 0) != hipSuccess || per < 1)
+        per = 1;
     CHK(upload_single(ds, a, a.nunits));
     BatchTri bt;
     bt.jobs = (const TriArgs *)ds->jobargs;
@@ -301,8 +319,7 @@ int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
     bt.njobs = 1;
     bt.total = a.nunits;
     bt.queue = a.queue;
-    if (tile == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(per * ds->cu), dim3(512), 0, s, bt);
-    else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(per * ds->cu), dim3(512), 0, s, bt);
+    launch_tri_mfma<MODE>(tile, per * ds->cu, bt, s);
     HIPCHK(hipGetLastError());
     return EPISDET_OK;
 }
@@ -811,9 +828,7 @@ unsigned long long resident_blocks(const episdet_dataset *ds, const FusedPlan &p
     const int cu = ds->cu;
     if (pl.order == 3) {
         if (pl.mfma)
-            return (unsigned long long)occupancy_blocks(pl.tile == 1 ? (const void *)k_triplets_mfma<MODE_SEARCH>
-                                                                     : (const void *)k_triplets_mfma2<MODE_SEARCH>,
-                                                        512, cu);
+            return (unsigned long long)occupancy_blocks(mfma_kernel<MODE_SEARCH>(pl.tile), mfma_threads(pl.tile), cu);
         const Tri3Tile t = TRI_TILES[pl.tile];
         if (t.A == 2 && t.B == 2) return (unsigned long long)launch_blocks((const void *)k_triplets_valu<2, 2, MODE_SEARCH>, cu);
         if (t.A == 4 && t.B == 2) return (unsigned long long)launch_blocks((const void *)k_triplets_valu<4, 2, MODE_SEARCH>, cu);
@@ -832,7 +847,8 @@ int tune_kernel(episdet_dataset *ds, int order, int score, std::vector<episdet_t
     std::vector<std::pair<int, int>> cands;
     if (mfma_ok(ds)) {
         cands.push_back({EPISDET_PATH_FUSED_MFMA, 0});
-        if (order == 3) cands.push_back({EPISDET_PATH_FUSED_MFMA, 1});
+        if (order == 3)
+            for (int t = 1; t < MFMA_TILES; t++) cands.push_back({EPISDET_PATH_FUSED_MFMA, t});
     }
     if (order == 3) {
         cands.push_back({EPISDET_PATH_FUSED_VALU, 0});
@@ -878,7 +894,8 @@ int tune_kernel(episdet_dataset *ds, int order, int score, std::vector<episdet_t
 
 // The kernel AUTO runs: the tuned choice for this shape if there is one (or
 // tune now when asked), else the FP4-MFMA kernels wherever they are exact
-// (fewer than 2^24 samples per class) and the VALU kernels otherwise.
+// (fewer than 2^24 samples per class, planes below 4 GiB per class) and the
+// VALU kernels otherwise.
 int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune)
 {
     const TuneKey key{order, score, ds->n, ds->W[0], ds->W[1]};
@@ -898,7 +915,11 @@ int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *til
         return EPISDET_OK;
     }
     *path = mfma_ok(ds) ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
-    *tile = 0;
+    // 3-way: with few sample chunks per work unit the epilogue weighs more,
+    // and version 3 (two workgroups per CU) hides it; measured at
+    // 8192 x 65536 (1024 words per class): 17.6 s vs 18.6 s; at
+    // 4096 x 262144 (4096 words): 7.67 s vs 7.61 s.
+    *tile = (*path == EPISDET_PATH_FUSED_MFMA && order == 3 && std::max(ds->W[0], ds->W[1]) <= 2048) ? 2 : 0;
     return EPISDET_OK;
 }
 
@@ -1399,7 +1420,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         int kpath = o.path, ktile = o.tile;
         if (o.path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, o.order, o.score, &kpath, &ktile, o.autotune != 0));
         const bool batchable =
-            kpath == EPISDET_PATH_FUSED_MFMA && mfma_ok(ds) && (ktile == 0 || (o.order == 3 && ktile == 1));
+            kpath == EPISDET_PATH_FUSED_MFMA && mfma_ok(ds) && (ktile == 0 || (o.order == 3 && ktile < MFMA_TILES));
         if (dup || !batchable) {
             solo.push_back(j);
             continue;
@@ -1495,8 +1516,6 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
 
     double gms = 0;
     if (!batched.empty()) {
-        const int blocks2 = occupancy_blocks((const void *)k_triplets_mfma2<MODE_SEARCH>, 512, cu);
-        const int blocks1 = occupancy_blocks((const void *)k_triplets_mfma<MODE_SEARCH>, 512, cu);
         const int blocksp = occupancy_blocks((const void *)k_pairs_mfma<MODE_SEARCH>, 512, cu);
         hipStream_t st, br[NQ];
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1523,8 +1542,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             bt.njobs = (int)tri[tl].size();
             bt.total = tstart[tl].back();
             bt.queue = d_bq + tl;
-            if (tl == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE_SEARCH>), dim3(blocks1), dim3(512), 0, br[tl], bt);
-            else hipLaunchKernelGGL((k_triplets_mfma2<MODE_SEARCH>), dim3(blocks2), dim3(512), 0, br[tl], bt);
+            launch_tri_mfma<MODE_SEARCH>(tl, occupancy_blocks(mfma_kernel<MODE_SEARCH>(tl), mfma_threads(tl), cu), bt,
+                                         br[tl]);
         }
         if (!pairs.empty()) {
             BatchPair bpr;
@@ -1665,7 +1684,7 @@ int episdet_tables(episdet_dataset *ds, int order, const int *sets, int m, unsig
     return EPISDET_OK;
 }
 
-static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *out, size_t capacity,
+static int dump_or_checksum(episdet_dataset *ds, int order, int path, int tile, unsigned *out, size_t capacity,
                             unsigned long long *checksum)
 {
     if (!ds || (order != 2 && order != 3)) return EPISDET_ERR_INVALID;
@@ -1673,6 +1692,7 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
         return EPISDET_ERR_INVALID;
     const bool mfma = path == EPISDET_PATH_FUSED_MFMA;
     if (mfma && !mfma_ok(ds)) return EPISDET_ERR_INVALID;
+    if (!tile_valid(order, mfma ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU, tile)) return EPISDET_ERR_INVALID;
     const int nc = order == 3 ? 27 : 9;
     const unsigned long long total = episdet_n_choose_k((unsigned)ds->n, (unsigned)order);
     uint32_t *d_dump = nullptr;
@@ -1689,9 +1709,9 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
         std::vector<int2> groups;
         std::vector<unsigned long long> gstart;
         if (mfma) {
-            build_groups(ds->n, mfma_ty(0), mfma_tz(0), groups, gstart);
+            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), groups, gstart);
         } else {
-            const Tri3Tile t = TRI_TILES[0];
+            const Tri3Tile t = TRI_TILES[tile];
             build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
         }
         if (hipMalloc(&d_gstart, sizeof(unsigned long long) * gstart.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
@@ -1707,17 +1727,17 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
         ta.nunits = gstart.back();
         ta.dump = d_dump;
         if (mfma)
-            rc = out ? dispatch_tri_mfma<MODE_DUMP>(ds, 0, ta, ds->stream)
-                     : dispatch_tri_mfma<MODE_CHECKSUM>(ds, 0, ta, ds->stream);
+            rc = out ? dispatch_tri_mfma<MODE_DUMP>(ds, tile, ta, ds->stream)
+                     : dispatch_tri_mfma<MODE_CHECKSUM>(ds, tile, ta, ds->stream);
         else
-            rc = out ? dispatch_tri<MODE_DUMP>(ds, 0, ta, 0, ds->stream)
-                     : dispatch_tri<MODE_CHECKSUM>(ds, 0, ta, 0, ds->stream);
+            rc = out ? dispatch_tri<MODE_DUMP>(ds, tile, ta, 0, ds->stream)
+                     : dispatch_tri<MODE_CHECKSUM>(ds, tile, ta, 0, ds->stream);
     } else {
         PairArgs pa;
-        fill_pairs(ds, pa, mfma, 0);
+        fill_pairs(ds, pa, mfma, tile);
         pa.dump = d_dump;
-        rc = out ? dispatch_pairs<MODE_DUMP>(ds, mfma, 0, pa, ds->stream)
-                 : dispatch_pairs<MODE_CHECKSUM>(ds, mfma, 0, pa, ds->stream);
+        rc = out ? dispatch_pairs<MODE_DUMP>(ds, mfma, tile, pa, ds->stream)
+                 : dispatch_pairs<MODE_CHECKSUM>(ds, mfma, tile, pa, ds->stream);
     }
     HIPCHK(hipStreamSynchronize(ds->stream));
     if (d_gstart) (void)hipFree(d_gstart);
@@ -1733,16 +1753,16 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, unsigned *
     return rc;
 }
 
-int episdet_dump_tables(episdet_dataset *ds, int order, int path, unsigned *out, size_t capacity)
+int episdet_dump_tables(episdet_dataset *ds, int order, int path, int tile, unsigned *out, size_t capacity)
 {
     if (!out) return EPISDET_ERR_INVALID;
-    return dump_or_checksum(ds, order, path, out, capacity, nullptr);
+    return dump_or_checksum(ds, order, path, tile, out, capacity, nullptr);
 }
 
-int episdet_checksum(episdet_dataset *ds, int order, int path, unsigned long long *checksum)
+int episdet_checksum(episdet_dataset *ds, int order, int path, int tile, unsigned long long *checksum)
 {
     if (!checksum) return EPISDET_ERR_INVALID;
-    return dump_or_checksum(ds, order, path, nullptr, 0, checksum);
+    return dump_or_checksum(ds, order, path, tile, nullptr, 0, checksum);
 }
 
 }  // extern "C"

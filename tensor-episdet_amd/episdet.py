@@ -153,9 +153,9 @@ def search_lib():
         lib.episdet_search_batch.restype = C.c_int
         lib.episdet_tables.argtypes = [C.c_void_p, C.c_int, _i32p, C.c_int, _u32p]
         lib.episdet_tables.restype = C.c_int
-        lib.episdet_dump_tables.argtypes = [C.c_void_p, C.c_int, C.c_int, _u32p, C.c_size_t]
+        lib.episdet_dump_tables.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _u32p, C.c_size_t]
         lib.episdet_dump_tables.restype = C.c_int
-        lib.episdet_checksum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_ulonglong)]
+        lib.episdet_checksum.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_ulonglong)]
         lib.episdet_checksum.restype = C.c_int
         lib.episdet_n_choose_k.argtypes = [C.c_uint, C.c_uint]
         lib.episdet_n_choose_k.restype = C.c_ulonglong
@@ -402,18 +402,18 @@ class Dataset:
             raise EpisdetError(rc, "episdet_tables")
         return out
 
-    def dump_tables(self, order, path=PATH_FUSED_VALU) -> np.ndarray:
+    def dump_tables(self, order, path=PATH_FUSED_VALU, tile=0) -> np.ndarray:
         total = n_choose_k(self.n, order)
         nc = 27 if order == 3 else 9
         out = np.zeros((total, 2, nc), dtype=np.uint32)
-        rc = self._lib.episdet_dump_tables(self._h, order, path, out, out.size)
+        rc = self._lib.episdet_dump_tables(self._h, order, path, tile, out, out.size)
         if rc:
             raise EpisdetError(rc, "episdet_dump_tables")
         return out
 
-    def checksum(self, order, path=PATH_FUSED_VALU) -> int:
+    def checksum(self, order, path=PATH_FUSED_VALU, tile=0) -> int:
         v = C.c_ulonglong()
-        rc = self._lib.episdet_checksum(self._h, order, path, C.byref(v))
+        rc = self._lib.episdet_checksum(self._h, order, path, tile, C.byref(v))
         if rc:
             raise EpisdetError(rc, "episdet_checksum")
         return v.value
