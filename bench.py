@@ -188,8 +188,9 @@ def main():
                     help="workload preset (BASELINE configs); 2 is the headline")
     ap.add_argument("--n", type=int, default=0, help="override the preset's SNP count")
     ap.add_argument("--samples", type=int, default=0, help="override the preset's sample count")
-    ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--path", choices=("mfma", "valu"), default="mfma")
+    ap.add_argument("--tile", type=int, default=0, help="kernel tile id (with --path mfma or valu)")
+    ap.add_argument("--path", choices=("auto", "mfma", "valu"), default="auto",
+                    help="auto: the library's choice for this shape (FP4 MFMA below 2^24 samples per class)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--plain-rounds", type=int, default=2048)
     ap.add_argument("--no-plain", action="store_true")
@@ -220,7 +221,7 @@ def main():
     ds = episdet.Dataset.from_synthetic(syn)
     sets = episdet.n_choose_k(n, order)
 
-    path = episdet.PATH_FUSED_MFMA if args.path == "mfma" else episdet.PATH_FUSED_VALU
+    path = {"auto": episdet.PATH_AUTO, "mfma": episdet.PATH_FUSED_MFMA, "valu": episdet.PATH_FUSED_VALU}[args.path]
 
     def step():
         return ds.run(order, score, path=path, tile=args.tile, rebuild=True)
@@ -241,7 +242,8 @@ def main():
     sec = max(times)
     value = sets * N / sec
     kernel_s = sum(kms) / len(kms) / 1e3
-    if args.path == "mfma":
+    ran = "mfma" if results[-1]["path"] == episdet.PATH_FUSED_MFMA else "valu"
+    if ran == "mfma":
         achieved = FLOPS_PER_SET_SAMPLE[order] * sets * N / kernel_s
         roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": MFMA_FP4_PEAK / 1e12,
                 "unit": f"TFLOP/s (dense FP4 MFMA, {FLOPS_PER_SET_SAMPLE[order]:.0f} flop per set-sample)", "frac": achieved / MFMA_FP4_PEAK}
@@ -250,7 +252,7 @@ def main():
         roof = {"bound": "valu", "achieved": achieved / 1e12, "peak": VALU_PEAK_OPS / 1e12,
                 "unit": f"T lane-ops/s (32-bit VALU, {OPS_PER_SET_SAMPLE[order]} per set-sample)", "frac": achieved / VALU_PEAK_OPS}
     r = results[-1]
-    roof.update({"traffic": measured_traffic(n, Nc, Nt, order, score, "fused_" + args.path, r["tile"]),
+    roof.update({"traffic": measured_traffic(n, Nc, Nt, order, score, "fused_" + ran, r["tile"]),
                  "kernel_ms": kernel_s * 1e3})
     planted_ok = (tuple(r["idx"][:order]) in syn.planted) if syn.planted else None
 
@@ -333,14 +335,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp4 e2m1 0/1 operands, fp32 exact-integer accumulation" if args.path == "mfma" else
+        "dtype": ("fp4 e2m1 0/1 operands, fp32 exact-integer accumulation" if ran == "mfma" else
                   "u32 bit-plane AND+popcount") + "; exact int counts, FP32 scoring, FP64 re-score",
         "data": "synthetic",
         "config": {"workload": f"{order}-way {SCORE_NAME[score]} exhaustive search, {n} SNPs x {N} samples "
                                f"({Nc}/{Nt}), HWE, MAF~{pre['maf_text']}, {pre['planted']} planted "
                                f"{'triplet' if order == 3 else 'pair'}(s), seed {args.seed}",
                    "preset": args.config,
-                   "snps": n, "cases": Nc, "controls": Nt, "sets": sets, "path": "fused_" + args.path,
+                   "snps": n, "cases": Nc, "controls": Nt, "sets": sets, "path": "fused_" + ran,
                    "tile": r["tile"], "parallelism": "1 GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
