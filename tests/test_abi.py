@@ -79,3 +79,16 @@ def test_invalid_arguments_rejected_before_device_use():
     # numSNPs < 2 and words too small for the sample count are argument errors
     assert lib.episdet_dataset_create(z, z, 1, 10, 10, 32, 32, ctypes.byref(h)) == 1
     assert lib.episdet_dataset_create(z, z, 4, 2000, 10, 32, 32, ctypes.byref(h)) == 1
+    # the extended entry points check their arguments before touching a device
+    res = episdet.Result()
+    opt = episdet.Options(order=3, score=0)
+    packed = (ctypes.c_ulonglong * 4)()
+    scores = (ctypes.c_double * 4)()
+    cnt = ctypes.c_int()
+    assert lib.episdet_run(None, ctypes.byref(opt), ctypes.byref(res)) == 1
+    assert lib.episdet_run_topk(None, ctypes.byref(opt), 4, packed, scores, ctypes.byref(cnt), ctypes.byref(res)) == 1
+    assert lib.episdet_autotune(None, 3, 0, None, 0, ctypes.byref(cnt), ctypes.byref(cnt)) == 1
+    assert lib.episdet_run_batch(None, ctypes.byref(opt), 0, ctypes.byref(res), None) == 1
+    assert lib.episdet_search_batch(None, 0, None, ctypes.byref(res), None) == 1
+    assert lib.episdet_dump_tables(None, 3, episdet.PATH_FUSED_MFMA, 0, np.zeros(4, dtype=np.uint32), 4) == 1
+    assert lib.episdet_checksum(None, 3, episdet.PATH_FUSED_MFMA, 0, None) == 1
