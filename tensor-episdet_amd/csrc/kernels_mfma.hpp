@@ -328,8 +328,23 @@ constexpr int M2_YROW = 2, M2_ZROW = 2 + 2 * M2_TY;
 // physical slot L%8 of row L/8, which holds logical slot (L%8) ^ (L/8).
 // Reads at or past num_records (the class's plane bytes) return zero.
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
-__device__ void m2_buffer_load_lds(i32x4_t rsrc, __attribute__((address_space(3))) uint32_t *lds, int size,
-                                   int voffset, int soffset, int offset, int aux) __asm("llvm.amdgcn.raw.buffer.load.lds");
+
+// One 16-byte-per-lane load from the buffer into LDS: lane L's data lands at
+// LDS byte address lds + 16 L.  Written as inline assembly on purpose: the
+// compiler cannot tell that these writes go to the other buffer of the
+// double-buffered stage, and for its own LDS-DMA intrinsic it waits for
+// them to finish before the next LDS read, which serialises every chunk's
+// loads with its contraction.  The kernels wait for these loads themselves
+// (s_waitcnt vmcnt(0) before the barrier that ends each chunk).
+__device__ __forceinline__ void lds_dma16(i32x4_t rsrc, const uint32_t *lds, unsigned voffset, unsigned soffset)
+{
+    const unsigned m0 = (unsigned)__builtin_amdgcn_readfirstlane(
+        (int)(unsigned)(size_t)(const __attribute__((address_space(3))) uint32_t *)lds);
+    asm volatile("s_mov_b32 m0, %0\n\tbuffer_load_dwordx4 %1, %2, %3 offen lds"
+                 :
+                 : "s"(m0), "v"(voffset), "s"(rsrc), "s"(soffset)
+                 : "memory");
+}
 
 constexpr int M2_LOADS_PER_WAVE = (M2_LOADS + 3) / 4;
 
@@ -370,9 +385,7 @@ __device__ __forceinline__ void m2_issue_buf(i32x4_t rsrc, const unsigned (&roff
 #pragma unroll
     for (int k = 0; k < M2_LOADS_PER_WAVE; k++) {
         const int ii = wih + 4 * k;
-        if (ii < M2_LOADS)
-            m2_buffer_load_lds(rsrc, (__attribute__((address_space(3))) uint32_t *)(buf + ii * 8 * M2_KW), 16,
-                               (int)roff[k], w0 * 4, 0, 0);
+        if (ii < M2_LOADS) lds_dma16(rsrc, buf + ii * 8 * M2_KW, roff[k], (unsigned)(w0 * 4));
     }
 }
 

@@ -59,9 +59,7 @@ __device__ __forceinline__ void m3_issue_buf(i32x4_t rsrc, const unsigned (&roff
 #pragma unroll
     for (int k = 0; k < M3_LOADS_PER_WAVE; k++) {
         const int ii = wih + 2 * k;
-        if (ii < M3_LOADS)
-            m2_buffer_load_lds(rsrc, (__attribute__((address_space(3))) uint32_t *)(buf + ii * M3_ROWS_PER_LOAD * M3_KW),
-                               16, (int)roff[k], w0 * 4, 0, 0);
+        if (ii < M3_LOADS) lds_dma16(rsrc, buf + ii * M3_ROWS_PER_LOAD * M3_KW, roff[k], (unsigned)(w0 * 4));
     }
 }
 

@@ -91,8 +91,7 @@ __device__ __forceinline__ void p2_issue_buf(i32x4_t rsrc, const unsigned (&roff
 #pragma unroll
     for (int k = 0; k < P2_LOADS_PER_WAVE; k++) {
         const int ii = wih + 4 * k;
-        m2_buffer_load_lds(rsrc, (__attribute__((address_space(3))) uint32_t *)(buf + ii * P2_ROWS_PER_LOAD * P2_KW),
-                           16, (int)roff[k], w0 * 4, 0, 0);
+        lds_dma16(rsrc, buf + ii * P2_ROWS_PER_LOAD * P2_KW, roff[k], (unsigned)(w0 * 4));
     }
 }
 
