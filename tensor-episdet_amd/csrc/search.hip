@@ -25,6 +25,7 @@
 #include "kernels_mfma.hpp"
 #include "kernels_mfma_pairs.hpp"
 #include "kernels_mfma3.hpp"
+#include "kernels_mfma4.hpp"
 
 using namespace episdet;
 
@@ -272,16 +273,19 @@ int dispatch_tri(episdet_dataset *ds, int tile, TriArgs &a, int grid_override, h
 
 // MFMA tiles: 0 = version 2 (64 x 64 units, one 512-thread workgroup per
 // CU), 1 = version 1 (32 x 64 units, register-staged chunks), 2 = version 3
-// (64 x 32 units, two 256-thread workgroups per CU).
-constexpr int MFMA_TILES = 3;
-inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : (tile == 2 ? M3_TY : M2_TY); }
-inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : (tile == 2 ? M3_TZ : M2_TZ); }
-inline int mfma_threads(int tile) { return tile == 2 ? 256 : 512; }
+// (64 x 32 units, two 256-thread workgroups per CU), 3 = version 4 (pairs of
+// X against 32 x 32, two workgroups per CU, triple-buffered chunks).
+constexpr int MFMA_TILES = 4;
+inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : (tile == 2 ? M3_TY : (tile == 3 ? M4_TY : M2_TY)); }
+inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : (tile == 2 ? M3_TZ : (tile == 3 ? M4_TZ : M2_TZ)); }
+inline int mfma_xstep(int tile) { return tile == 3 ? 2 : 1; }   // X values per work unit
+inline int mfma_threads(int tile) { return tile >= 2 ? 256 : 512; }
 template <int MODE>
 const void *mfma_kernel(int tile)
 {
     if (tile == 1) return (const void *)k_triplets_mfma<MODE>;
     if (tile == 2) return (const void *)k_triplets_mfma3<MODE>;
+    if (tile == 3) return (const void *)k_triplets_mfma4<MODE>;
     return (const void *)k_triplets_mfma2<MODE>;
 }
 template <int MODE>
@@ -289,6 +293,7 @@ void launch_tri_mfma(int tile, int blocks, const BatchTri &bt, hipStream_t s)
 {
     if (tile == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(blocks), dim3(512), 0, s, bt);
     else if (tile == 2) hipLaunchKernelGGL((k_triplets_mfma3<MODE>), dim3(blocks), dim3(256), 0, s, bt);
+    else if (tile == 3) hipLaunchKernelGGL((k_triplets_mfma4<MODE>), dim3(blocks), dim3(256), 0, s, bt);
     else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(blocks), dim3(512), 0, s, bt);
 }
 
@@ -383,7 +388,8 @@ int dispatch_pairs(episdet_dataset *ds, bool mfma, int tile, PairArgs &a, hipStr
 
 // Groups of 3-way units: (Y tile, Z tile) pairs that hold at least one
 // X < Y < Z, each with the number of X values it needs.
-void build_groups(int n, int TY, int TZ, std::vector<int2> &groups, std::vector<unsigned long long> &gstart)
+void build_groups(int n, int TY, int TZ, std::vector<int2> &groups, std::vector<unsigned long long> &gstart,
+                  int xstep = 1)
 {
     groups.clear();
     gstart.clear();
@@ -398,7 +404,7 @@ void build_groups(int n, int TY, int TZ, std::vector<int2> &groups, std::vector<
             if (xcount <= 0) continue;
             groups.push_back(make_int2(ty, tz));
             gstart.push_back(acc);
-            acc += (unsigned long long)xcount;
+            acc += (unsigned long long)((xcount + xstep - 1) / xstep);
         }
     }
     gstart.push_back(acc);
@@ -658,7 +664,7 @@ int plan_fused(episdet_dataset *ds, int order, int score, int path, int tile, Fu
     pl.sc = make_ctx(ds, score, order);
     if (order == 3) {
         if (pl.mfma) {
-            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), pl.groups, pl.gstart);
+            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), pl.groups, pl.gstart, mfma_xstep(tile));
         } else {
             const Tri3Tile t = TRI_TILES[tile];
             build_groups(ds->n, 16 * t.A, 16 * t.B, pl.groups, pl.gstart);
@@ -787,16 +793,17 @@ unsigned long long count_sets(const episdet_dataset *ds, const FusedPlan &pl, un
     if (pl.order == 3) {
         const long long TY = pl.mfma ? mfma_ty(pl.tile) : 16 * TRI_TILES[pl.tile].A;
         const long long TZ = pl.mfma ? mfma_tz(pl.tile) : 16 * TRI_TILES[pl.tile].B;
+        const long long XS = pl.mfma ? mfma_xstep(pl.tile) : 1;
         size_t g = (size_t)(std::upper_bound(pl.gstart.begin(), pl.gstart.end(), u0) - pl.gstart.begin()) - 1;
         for (unsigned long long u = u0; u < u1; u++) {
             while (g + 1 < pl.groups.size() && pl.gstart[g + 1] <= u) g++;
-            const long long X = (long long)(u - pl.gstart[g]);
             const long long ylo = (long long)pl.groups[g].x * TY, zlo = (long long)pl.groups[g].y * TZ;
             const long long zhi = std::min(zlo + TZ, n);
-            for (long long Y = std::max(ylo, X + 1); Y < std::min(ylo + TY, n); Y++) {
-                const long long z0 = std::max(zlo, Y + 1);
-                if (zhi > z0) total += (unsigned long long)(zhi - z0);
-            }
+            for (long long X = XS * (long long)(u - pl.gstart[g]); X < XS * (long long)(u - pl.gstart[g] + 1); X++)
+                for (long long Y = std::max(ylo, X + 1); Y < std::min(ylo + TY, n); Y++) {
+                    const long long z0 = std::max(zlo, Y + 1);
+                    if (zhi > z0) total += (unsigned long long)(zhi - z0);
+                }
         }
     } else {
         const long long TA = pl.mfma ? P2_TA : 16 * PAIR_TILES[pl.tile];
@@ -915,11 +922,10 @@ int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *til
         return EPISDET_OK;
     }
     *path = mfma_ok(ds) ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
-    // 3-way: with few sample chunks per work unit the epilogue weighs more,
-    // and version 3 (two workgroups per CU) hides it; measured at
-    // 8192 x 65536 (1024 words per class): 17.6 s vs 18.6 s; at
-    // 4096 x 262144 (4096 words): 7.67 s vs 7.61 s.
-    *tile = (*path == EPISDET_PATH_FUSED_MFMA && order == 3 && std::max(ds->W[0], ds->W[1]) <= 2048) ? 2 : 0;
+    // 3-way: version 4 (pairs of X, two workgroups per CU) measured fastest
+    // at both 4096 x 262144 (7.01 s against 7.27 s for version 2) and
+    // 8192 x 65536 (16.8 s against 17.0 s for version 3, 18.4 s for 2).
+    *tile = (*path == EPISDET_PATH_FUSED_MFMA && order == 3) ? 3 : 0;
     return EPISDET_OK;
 }
 
@@ -1441,7 +1447,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             episdet_dataset *ds = dss[tri[tl][k]];
             std::vector<int2> groups;
             std::vector<unsigned long long> gstart;
-            build_groups(ds->n, mfma_ty(tl), mfma_tz(tl), groups, gstart);
+            build_groups(ds->n, mfma_ty(tl), mfma_tz(tl), groups, gstart, mfma_xstep(tl));
             unsigned long long *d_gstart;
             int2 *d_groups;
             CHK(dl.put(&d_gstart, gstart.data(), gstart.size()));
@@ -1709,7 +1715,7 @@ static int dump_or_checksum(episdet_dataset *ds, int order, int path, int tile, 
         std::vector<int2> groups;
         std::vector<unsigned long long> gstart;
         if (mfma) {
-            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), groups, gstart);
+            build_groups(ds->n, mfma_ty(tile), mfma_tz(tile), groups, gstart, mfma_xstep(tile));
         } else {
             const Tri3Tile t = TRI_TILES[tile];
             build_groups(ds->n, 16 * t.A, 16 * t.B, groups, gstart);
