@@ -39,8 +39,8 @@ FLOPS_PER_SET_SAMPLE = {2: 8.0, 3: 16.0} # MFMA: 2^k multiply-adds per set-sampl
 
 
 SCORE_NAME = {0: "K2", 1: "MI"}
-# BASELINE configs 1, 2, 3 and 5 (config 4, the batched jobs, is not a
-# single search).  Config 3's classes are 26214 / 39322 of 65536.
+# Single-search workload presets 1, 2, 3 and 5 (preset 4, the 64-job batch,
+# is not a single search).  Preset 3's classes are 26214 / 39322 of 65536.
 PRESETS = {
     1: dict(order=2, score=0, n=512, samples=2048, case_frac=0.5, planted=0, maf=(0.05, 0.5), maf_dist=0,
             maf_text="U[0.05,0.5]"),
@@ -74,7 +74,7 @@ def measured_traffic(n, Nc, Nt, order, score, path, tile):
 
 
 def batch_jobs(seed):
-    """64 independent jobs; order and score drawn 50/50,
+    """Preset 4, the 64-job batch: 64 independent jobs; order and score drawn 50/50,
     SNPs log-uniform in [256, 8192] (3-way in [256, 2048]), samples uniform in
     [8000, 131072] split 50/50, packed tight (ceil(N/32) words, not the
     1024-sample padding), MAF ~ U[0.05, 0.5]."""
@@ -185,7 +185,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--config", type=int, default=2, choices=(1, 2, 3, 4, 5),
-                    help="workload preset (BASELINE configs); 2 is the headline")
+                    help="workload preset; 2 (3-way K2, 4096 SNPs x 262144 samples) is the headline")
     ap.add_argument("--n", type=int, default=0, help="override the preset's SNP count")
     ap.add_argument("--samples", type=int, default=0, help="override the preset's sample count")
     ap.add_argument("--tile", type=int, default=0, help="kernel tile id (with --path mfma or valu)")
@@ -199,7 +199,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the search shards with no "
-              "communication, see DESIGN.md", file=sys.stderr)
+              "communication between GPUs (each GPU takes a contiguous range of work units)", file=sys.stderr)
         sys.exit(2)
 
     import episdet

@@ -335,7 +335,11 @@ typedef int i32x4_t __attribute__((ext_vector_type(4)));
 // double-buffered stage, and for its own LDS-DMA intrinsic it waits for
 // them to finish before the next LDS read, which serialises every chunk's
 // loads with its contraction.  The kernels wait for these loads themselves
-// (s_waitcnt vmcnt(0) before the barrier that ends each chunk).
+// (s_waitcnt vmcnt before the barrier that ends each chunk).  m0 is written
+// without being named as clobbered (the compiler treats m0 as reserved and
+// does not honour it in a clobber list); that is sound because no compiler-generated code in
+// these kernels uses m0, which tests/test_device_asm.py checks on the
+// assembly the Makefile's flags produce.
 __device__ __forceinline__ void lds_dma16(i32x4_t rsrc, const uint32_t *lds, unsigned voffset, unsigned soffset)
 {
     const unsigned m0 = (unsigned)__builtin_amdgcn_readfirstlane(
