@@ -41,7 +41,9 @@ enum episdet_status {
     EPISDET_ERR_ALLOC = 4,        /* device or host allocation failed               */
     EPISDET_ERR_CANDIDATES = 5,   /* candidate buffer overflow that could not be
                                      resolved by the exact fallback               */
-    EPISDET_ERR_INTERNAL = 6
+    EPISDET_ERR_INTERNAL = 6,
+    EPISDET_ERR_GENOTYPES = 7     /* a sample is set in both planes of a SNP: the
+                                     input encodes no genotype for it             */
 };
 
 enum episdet_score { EPISDET_SCORE_K2 = 0, EPISDET_SCORE_MI = 1 };

@@ -169,9 +169,27 @@ __device__ __forceinline__ float badness_f32(const ScoreCtx &sc, const uint32_t 
 // ---------------------------------------------------------------- FP64 --
 // Fixed operation sequences, no contraction: identical to the oracle's.
 
-__device__ __forceinline__ double k2_f64(const uint32_t *cas, const uint32_t *ctl, int ncells, const double *T)
+// The FP64 scores index their tables (T[0..N+1], L[0..N]) by the cells.
+// Cells derived from consistent count tables each lie within their class
+// and sum to its size; a table that does not is scored NaN instead of being
+// looked up (the host reports a NaN score as an internal error).
+__device__ __forceinline__ bool cells_valid(const uint32_t *cas, const uint32_t *ctl, int ncells, int Nc, int Nt)
+{
+    unsigned long long sc = 0, st = 0;
+    bool ok = true;
+    for (int i = 0; i < ncells; i++) {
+        ok = ok && cas[i] <= (uint32_t)Nc && ctl[i] <= (uint32_t)Nt;
+        sc += cas[i];
+        st += ctl[i];
+    }
+    return ok && sc == (unsigned long long)Nc && st == (unsigned long long)Nt;
+}
+
+__device__ __forceinline__ double k2_f64(const uint32_t *cas, const uint32_t *ctl, int ncells, int Nc, int Nt,
+                                         const double *T)
 {
 #pragma clang fp contract(off)
+    if (!cells_valid(cas, ctl, ncells, Nc, Nt)) return __builtin_nan("");
     double s = 0.0;
     for (int i = 0; i < ncells; i++) {
         double t = T[ctl[i]] + T[cas[i]];
@@ -185,6 +203,7 @@ __device__ __forceinline__ double mi_f64(const uint32_t *cas, const uint32_t *ct
                                          const double *L)
 {
 #pragma clang fp contract(off)
+    if (!cells_valid(cas, ctl, ncells, Nc, Nt)) return __builtin_nan("");
     double invN = 1.0 / (double)(Nc + Nt);
     double hY = -((double)Nc * invN) * L[Nc];
     hY = hY - ((double)Nt * invN) * L[Nt];

@@ -47,24 +47,36 @@ __global__ void k_repack(const uint32_t *__restrict__ src, int Win, uint32_t *__
 
 // Single-SNP plane counts: one wave per SNP, both classes
 // (helper.cu:13-52).  out[v] = {cases c0, cases c1, controls c0, controls c1}.
+// A sample marked in both planes of a SNP encodes no genotype, and every
+// derived count of that SNP would be off (genotype 2 = N - c0 - c1 can go
+// negative); such SNPs lower *bad_snp (initialised to ~0) to their index.
 __global__ __launch_bounds__(64) void k_singles(const uint32_t *__restrict__ p0, int W0,
-                                                const uint32_t *__restrict__ p1, int W1, int n, uint4 *out)
+                                                const uint32_t *__restrict__ p1, int W1, int n, uint4 *out,
+                                                unsigned *bad_snp)
 {
     const int v = blockIdx.x, lane = threadIdx.x;
     if (v >= n) return;
-    uint32_t c[4] = {0, 0, 0, 0};
+    uint32_t c[4] = {0, 0, 0, 0}, both = 0;
     const uint32_t *r0 = p0 + (size_t)v * 2 * W0, *r1 = p1 + (size_t)v * 2 * W1;
     for (int w = lane; w < W0; w += 64) {
-        c[0] += __popc(r0[w]);
-        c[1] += __popc(r0[W0 + w]);
+        const uint32_t a = r0[w], b = r0[W0 + w];
+        c[0] += __popc(a);
+        c[1] += __popc(b);
+        both |= a & b;
     }
     for (int w = lane; w < W1; w += 64) {
-        c[2] += __popc(r1[w]);
-        c[3] += __popc(r1[W1 + w]);
+        const uint32_t a = r1[w], b = r1[W1 + w];
+        c[2] += __popc(a);
+        c[3] += __popc(b);
+        both |= a & b;
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) c[i] = wave_sum_u(c[i]);
-    if (lane == 0) out[v] = make_uint4(c[0], c[1], c[2], c[3]);
+    const uint32_t nboth = wave_sum_u(both ? 1u : 0u);
+    if (lane == 0) {
+        out[v] = make_uint4(c[0], c[1], c[2], c[3]);
+        if (nboth) atomicMin(bad_snp, (unsigned)v);
+    }
 }
 
 // -------------------------------------------------------- shared output --
@@ -642,7 +654,7 @@ __global__ __launch_bounds__(256) void k_triplets_objective_plain(PlainObjArgs p
         pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 1, (uint32_t)p.N1, xz);
         pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 1, (uint32_t)p.N1, yz);
         triplet27(n8t, xy, xz, yz, ctl);
-        bad = p.score == SCORE_K2 ? k2_f64(cas, ctl, 27, p.t64) : -mi_f64(cas, ctl, 27, p.N0, p.N1, p.t64);
+        bad = p.score == SCORE_K2 ? k2_f64(cas, ctl, 27, p.N0, p.N1, p.t64) : -mi_f64(cas, ctl, 27, p.N0, p.N1, p.t64);
     }
     const double wmin = wave_min_d(bad);
     const int tid = threadIdx.x;
@@ -658,7 +670,9 @@ __global__ __launch_bounds__(256) void k_triplets_objective_plain(PlainObjArgs p
         s_red[4] = thr;
     }
     __syncthreads();
-    if (valid && bad <= s_red[4]) {
+    // a NaN score (inconsistent tables, see cells_valid) is published too,
+    // so that the host reports it instead of skipping the set
+    if (valid && (bad <= s_red[4] || bad != bad)) {
         unsigned slot = atomicAdd(p.cand_count, 1u);
         if (slot < p.cand_cap) {
             p.cand[slot] = (unsigned long long)X | ((unsigned long long)Y << 21) | ((unsigned long long)Z << 42);
@@ -752,7 +766,8 @@ __device__ __forceinline__ void rescore_one(const RescoreArgs &p, int i)
         }
     }
     if (p.out_s64)
-        p.out_s64[i] = p.score == SCORE_K2 ? k2_f64(cas, ctl, nc, p.t64) : mi_f64(cas, ctl, nc, p.N0, p.N1, p.t64);
+        p.out_s64[i] =
+            p.score == SCORE_K2 ? k2_f64(cas, ctl, nc, p.N0, p.N1, p.t64) : mi_f64(cas, ctl, nc, p.N0, p.N1, p.t64);
 }
 
 __global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)

@@ -71,6 +71,7 @@ struct Ctl {                  // device control block
     unsigned long long checksum;
     unsigned best32;
     unsigned cand_count;
+    unsigned bad_snp;         // lowest SNP with a sample in both planes (k_singles), or ~0
 };
 
 }  // namespace
@@ -575,6 +576,7 @@ int collect_best(episdet_dataset *ds, int order, int score, int count, episdet_r
     float b32 = 0.f;
     double maxabs = 0, maxrel = 0;
     for (int i = 0; i < count; i++) {
+        if (std::isnan(h64[i])) return EPISDET_ERR_INTERNAL;   // inconsistent count tables
         const double s32 = score == EPISDET_SCORE_K2 ? (double)h32[i] : -(double)h32[i];
         const double err = std::fabs(s32 - h64[i]);
         maxabs = std::max(maxabs, err);
@@ -1110,6 +1112,7 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     double bb = INFINITY;
     unsigned long long bp = ~0ull;
     for (int i = 0; i < count; i++) {
+        if (std::isnan(h64[i])) return EPISDET_ERR_INTERNAL;   // inconsistent count tables
         if (h64[i] < bb || (h64[i] == bb && hp[i] < bp)) {
             bb = h64[i];
             bp = hp[i];
@@ -1176,6 +1179,7 @@ const char *episdet_status_string(int status)
     case EPISDET_ERR_HIP: return "HIP runtime error";
     case EPISDET_ERR_ALLOC: return "allocation failed";
     case EPISDET_ERR_CANDIDATES: return "candidate buffer overflow";
+    case EPISDET_ERR_GENOTYPES: return "a sample is marked in both genotype planes of a SNP";
     default: return "internal error";
     }
 }
@@ -1194,12 +1198,21 @@ int episdet_device_info(char *name, int name_len, int *compute_units, int *clock
     return EPISDET_OK;
 }
 
+// Single counts, and the check that no sample is marked in both genotype
+// planes of a SNP (such input has no genotype for that sample, and the
+// derived counts of the SNP would be inconsistent).  Never captured: it
+// synchronises.
 static int compute_singles(episdet_dataset *ds)
 {
+    unsigned *bad = &ds->ctl->bad_snp;
+    HIPCHK(hipMemsetAsync(bad, 0xFF, sizeof(unsigned), ds->stream));
     hipLaunchKernelGGL(k_singles, dim3(ds->n), dim3(64), 0, ds->stream, ds->planes[0], ds->W[0], ds->planes[1],
-                       ds->W[1], ds->n, ds->singles);
+                       ds->W[1], ds->n, ds->singles, bad);
     HIPCHK(hipGetLastError());
+    unsigned hb = 0;
+    HIPCHK(hipMemcpyAsync(&hb, bad, sizeof hb, hipMemcpyDeviceToHost, ds->stream));
     HIPCHK(hipStreamSynchronize(ds->stream));
+    if (hb != ~0u) return EPISDET_ERR_GENOTYPES;
     return EPISDET_OK;
 }
 

@@ -28,7 +28,8 @@ TOPK_MAX = 1024   # EPISDET_TOPK_MAX
 MAF_UNIFORM, MAF_BETA = 0, 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "no gfx950 device", 3: "HIP runtime error",
-          4: "allocation failed", 5: "candidate buffer overflow", 6: "internal error"}
+          4: "allocation failed", 5: "candidate buffer overflow", 6: "internal error",
+          7: "a sample is marked in both genotype planes of a SNP"}
 
 
 class NativeMissing(RuntimeError):

@@ -62,6 +62,7 @@ def test_host_helpers():
     lib = episdet.search_lib()
     assert lib.episdet_status_string(0) == b"ok"
     assert lib.episdet_status_string(2) == b"no gfx950 device"
+    assert lib.episdet_status_string(7) == episdet.STATUS[7].encode()
 
 
 def test_cli_binaries_built():

@@ -329,6 +329,31 @@ def test_topk_more_than_all_sets_and_errors():
             ds.topk(3, episdet.TOPK_MAX + 1)
 
 
+def test_sample_in_both_planes_is_rejected():
+    """A sample marked in both genotype planes of a SNP encodes no genotype
+    (its genotype-2 count N - c0 - c1 would be off): dataset creation and the
+    one-shot search refuse it with EPISDET_ERR_GENOTYPES.  Both bits set past
+    the last sample are padding, cleared on upload, and accepted."""
+    syn = episdet.generate(20, 300, 200, seed=3)
+    bad = syn.cases.copy()
+    bad[7, 0, 2] |= 0x00010000          # sample 2 * 32 + 15 of SNP 7, in both planes
+    bad[7, 1, 2] |= 0x00010000
+    say("overlapping planes")
+    with pytest.raises(episdet.EpisdetError) as e:
+        episdet.Dataset(bad, syn.controls, 300, 200)
+    assert e.value.code == 7
+    with pytest.raises(episdet.EpisdetError) as e:
+        episdet.search(3, episdet.K2, bad, syn.controls, 300, 200)
+    assert e.value.code == 7
+    pad = syn.controls.copy()
+    pad[3, 0, 6] |= 1                   # sample 6 * 32 + 31 = 223 >= 200: padding
+    pad[3, 1, 6] |= 1
+    with episdet.Dataset(syn.cases, pad, 300, 200) as ds:
+        r = ds.run(3, episdet.K2)
+    o = O.search(syn.cases, syn.controls, 300, 200, 3, O.K2)
+    assert r["packed"] == o["packed"] and r["best_score"] == o["score"]
+
+
 # ---------------------------------------------------------- autotuner ----
 
 @pytest.mark.parametrize("order,n,Nc,Nt", [(3, 150, 3000, 2500), (2, 700, 5000, 4000)])
