@@ -1,5 +1,5 @@
 // kernels_mfma4.hpp -- fused 3-way search, FP4 MFMA, two workgroups per CU,
-// pairs of X per work unit, triple-buffered chunks.
+// pairs of X per work unit, multi-buffered chunks.
 //
 // Same contraction, FP4 encoding and epilogue as versions 2 and 3.  Version
 // 3 hides the epilogue by running two workgroups per CU but re-streams its Z
@@ -10,46 +10,64 @@
 //  * 256 threads = 4 waves: waves 0-1 the cases, 2-3 the controls; wave
 //    (class, xs) owns X + xs against 32 Y x 32 Z = 4 x 2 MFMA tiles (128
 //    accumulators);
-//  * a 512-sample chunk of a class is 4 X rows + 64 Y rows + 64 Z rows of
-//    64 bytes (132 rows for 4096 triplet-samples per class, against 194 in
-//    version 3), loaded straight to LDS through the scalar buffer
-//    descriptor into one of three buffers, two chunks ahead;
+//  * a chunk of a class is 4 X rows + 64 Y rows + 64 Z rows (132 rows for
+//    4096 triplet-samples per class, against 194 in version 3), loaded
+//    straight to LDS through the scalar buffer descriptor, NBUF - 1 chunks
+//    ahead.  KW = 16 (tile 3): 512-sample chunks, three buffers of 9 KB per
+//    class.  KW = 32 (tile 4): 1024-sample chunks, two buffers of 17 KB, so
+//    half as many barriers and load rounds per sample;
 //  * after the last chunk the accumulators go to LDS in two halves (Z 0-15,
 //    then 16-31) and the 256 threads score four triplets per half.
-// LDS: 64 KB for the stage (3 x 2 buffers of 9 KB) / dump area plus the
-// ln m! table, so two workgroups fit a CU.
+// LDS: at most 68 KB for the stage / dump area plus the ln m! table, so two
+// workgroups fit a CU.
 #pragma once
 #include "kernels_mfma3.hpp"
 
 namespace episdet {
 
 constexpr int M4_TY = 32, M4_TZ = 32;
-constexpr int M4_KW = 16;                                   // words per chunk (512 samples)
 constexpr int M4_YROW = 4, M4_ZROW = 4 + 2 * M4_TY;         // X rows 0-3, then Y, then Z
 constexpr int M4_ROWS = M4_ZROW + 2 * M4_TZ;                // 132 rows of a class
-constexpr int M4_ROWS_PER_LOAD = 16;                        // 64-byte rows: one load = 1 KB
-constexpr int M4_LOADS = (M4_ROWS + M4_ROWS_PER_LOAD - 1) / M4_ROWS_PER_LOAD;   // 9
-constexpr int M4_BUF_WORDS = M4_LOADS * M4_ROWS_PER_LOAD * M4_KW;              // 2304 (9 KB)
-constexpr int M4_NBUF = 3;
-constexpr int M4_STAGE_WORDS = 2 * M4_NBUF * M4_BUF_WORDS;
 constexpr int M4_DUMP_WORDS = 2 * 2 * 4 * 16 * 64;          // class x wave x tile x reg x lane (64 KB)
-constexpr int M4_SMEM_WORDS = M4_STAGE_WORDS > M4_DUMP_WORDS ? M4_STAGE_WORDS : M4_DUMP_WORDS;
-constexpr int M4_LOADS_PER_WAVE = (M4_LOADS + 1) / 2;       // two waves per class: 5 and 4 loads
-static_assert(M4_KW == 16, "swizzle assumes four 16-byte slots per row");
+
+// Chunk geometry, by words per row per chunk (KW):
+//  * KW 16 (512 samples): 64-byte rows, 16 rows per load, three buffers;
+//  * KW 32 (1024 samples): 128-byte rows, 8 rows per load, two buffers,
+//    half the chunks (and barriers) per sample.
+// Both keep two workgroups per CU (stage and dump area <= 68 KB).
+template <int KW>
+struct M4Geo {
+    static_assert(KW == 16 || KW == 32, "chunk of 16 or 32 words");
+    static constexpr int SLOTS = KW / 4;                     // 16-byte slots per row
+    static constexpr int ROWS_PER_LOAD = 64 / SLOTS;         // one load = 1 KB
+    static constexpr int LOADS = (M4_ROWS + ROWS_PER_LOAD - 1) / ROWS_PER_LOAD;   // 9 / 17
+    static constexpr int LOADS_PER_WAVE = (LOADS + 1) / 2;   // two waves per class
+    static constexpr int BUF_WORDS = LOADS * ROWS_PER_LOAD * KW;
+    static constexpr int NBUF = KW == 16 ? 3 : 2;
+    static constexpr int STAGE_WORDS = 2 * NBUF * BUF_WORDS;
+    static constexpr int SMEM_WORDS = STAGE_WORDS > M4_DUMP_WORDS ? STAGE_WORDS : M4_DUMP_WORDS;
+    // swizzle: logical slot q of row r sits at physical slot q ^ swz(r), so
+    // the rows one ds_read_b128 touches spread over every slot
+    static __device__ __forceinline__ int swz(int r) { return KW == 16 ? ((r >> 1) & 3) : (r & 7); }
+    static __device__ __forceinline__ int at(int r, int q) { return r * KW + ((q ^ swz(r)) << 2); }
+};
 
 // s_waitcnt immediates (gfx9 layout: vmcnt [3:0] and [15:14], expcnt [6:4],
 // lgkmcnt [11:8]) that wait for vector-memory loads only.
 constexpr int m4_wait_vm(int n) { return (n & 0xF) | (0x7 << 4) | (0xF << 8) | ((n >> 4) << 14); }
 
 // Byte offsets, within the class's planes, of slot q of each LDS row this
-// lane loads (load k of wave wih fills rows 16(wih + 2k) .. +15).
+// lane loads (load k of wave wih fills rows RPL(wih + 2k) .. +RPL-1).
+template <int KW>
 __device__ __forceinline__ void m4_row_offsets(int W, int X0, int y0, int z0, int wih, int lane,
-                                               unsigned (&roff)[M4_LOADS_PER_WAVE])
+                                               unsigned (&roff)[M4Geo<KW>::LOADS_PER_WAVE])
 {
-    const int rl = lane >> 2, q = (lane & 3) ^ ((rl >> 1) & 3);
+    using G = M4Geo<KW>;
+    const int rl = lane / G::SLOTS;
 #pragma unroll
-    for (int k = 0; k < M4_LOADS_PER_WAVE; k++) {
-        const int r = (wih + 2 * k) * M4_ROWS_PER_LOAD + rl;
+    for (int k = 0; k < G::LOADS_PER_WAVE; k++) {
+        const int r = (wih + 2 * k) * G::ROWS_PER_LOAD + rl;
+        const int q = (lane % G::SLOTS) ^ G::swz(r);
         int row;
         if (r < M4_YROW) row = 2 * X0 + r;
         else if (r < M4_ZROW) row = 2 * y0 + (r - M4_YROW);
@@ -59,23 +77,23 @@ __device__ __forceinline__ void m4_row_offsets(int W, int X0, int y0, int z0, in
     }
 }
 
-__device__ __forceinline__ void m4_issue_buf(i32x4_t rsrc, const unsigned (&roff)[M4_LOADS_PER_WAVE], int w0,
+template <int KW>
+__device__ __forceinline__ void m4_issue_buf(i32x4_t rsrc, const unsigned (&roff)[M4Geo<KW>::LOADS_PER_WAVE], int w0,
                                              uint32_t *buf, int wih)
 {
+    using G = M4Geo<KW>;
 #pragma unroll
-    for (int k = 0; k < M4_LOADS_PER_WAVE; k++) {
+    for (int k = 0; k < G::LOADS_PER_WAVE; k++) {
         const int ii = wih + 2 * k;
-        if (ii < M4_LOADS) lds_dma16(rsrc, buf + ii * M4_ROWS_PER_LOAD * M4_KW, roff[k], (unsigned)(w0 * 4));
+        if (ii < G::LOADS) lds_dma16(rsrc, buf + ii * G::ROWS_PER_LOAD * KW, roff[k], (unsigned)(w0 * 4));
     }
 }
 
-// Number of groups' units with X stepped by two: the unit list of version
-// 4 is build_groups' with each group's X count halved (rounded up).
-
-template <int MODE>
+template <int MODE, int KW>
 __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
 {
-    __shared__ __attribute__((aligned(16))) uint32_t smem[M4_SMEM_WORDS];
+    using G = M4Geo<KW>;
+    __shared__ __attribute__((aligned(16))) uint32_t smem[G::SMEM_WORDS];
     __shared__ unsigned long long s_u;
     __shared__ float s_red[8];
     __shared__ float s_lnf[scores_sets(MODE) ? LNF_SMALL : 1];
@@ -83,7 +101,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
     const int tid = threadIdx.x;
     const int cls = tid >> 7, t = tid & 127;
     const int xs = t >> 6, lane = t & 63, h = lane >> 5, i = lane & 31;
-    uint32_t *bufs = smem + cls * M4_NBUF * M4_BUF_WORDS;
+    uint32_t *bufs = smem + cls * G::NBUF * G::BUF_WORDS;
     // LDS rows read by this lane: its X's plane, Y rows of the four row
     // blocks, Z rows of the two column blocks
     const int xr = 2 * xs + ((i >> 1) & 1);
@@ -92,7 +110,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
     for (int rb = 0; rb < 4; rb++) yr[rb] = M4_YROW + 2 * (rb * 8 + (i >> 2)) + (i & 1);
     const int zr0 = M4_ZROW + i, zr1 = M4_ZROW + 32 + i;
     // loads per chunk issued by this wave
-    const int my_loads = (M4_LOADS - xs + 1) / 2;
+    const int my_loads = (G::LOADS - xs + 1) / 2;
 
     int g = 0, j = 0;
     for (;;) {
@@ -111,15 +129,15 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
         const unsigned long long u = uu - bt.job_start[j];
         const uint32_t *base = cls ? p.p1 : p.p0;
         const int W = cls ? p.W1 : p.W0;
-        const int nch_mine = W / M4_KW;
-        const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / M4_KW;
+        const int nch_mine = W / KW;
+        const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / KW;
         g = next_group(p.gstart, p.ngroups, u, g);
         const int X0 = 2 * (int)(u - p.gstart[g]);
         const int2 gt = p.groups[g];
         const int y0 = gt.x * M4_TY, z0 = gt.y * M4_TZ;
         const i32x4_t rsrc = m2_rsrc(base, cls ? p.bytes1 : p.bytes0);
-        unsigned roff[M4_LOADS_PER_WAVE];
-        m4_row_offsets(W, X0, y0, z0, xs, lane, roff);
+        unsigned roff[G::LOADS_PER_WAVE];
+        m4_row_offsets<KW>(W, X0, y0, z0, xs, lane, roff);
 
         v16f_t acc[4][2];
 #pragma unroll
@@ -129,33 +147,36 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
 
-        // chunks 0 and 1 in flight; chunk c + 2 goes out once every wave is
-        // past chunk c - 1, whose buffer it reuses
-        if (nch_mine > 0) m4_issue_buf(rsrc, roff, 0, bufs, xs);
-        if (nch_mine > 1) m4_issue_buf(rsrc, roff, M4_KW, bufs + M4_BUF_WORDS, xs);
+        // NBUF - 1 chunks in flight; chunk c + NBUF - 1 goes out once every
+        // wave is past chunk c - 1, whose buffer it reuses
+#pragma unroll
+        for (int k = 0; k + 1 < G::NBUF; k++)
+            if (k < nch_mine) m4_issue_buf<KW>(rsrc, roff, k * KW, bufs + k * G::BUF_WORDS, xs);
         for (int c = 0; c < nch; c++) {
-            // this wave's loads of chunk c are done once at most chunk c+1's
-            // are outstanding
-            if (c + 1 < nch_mine) {
-                if (my_loads == 5) __builtin_amdgcn_s_waitcnt(m4_wait_vm(5));
-                else __builtin_amdgcn_s_waitcnt(m4_wait_vm(4));
+            // this wave's loads of chunk c are done once at most those of the
+            // chunks after it that are already issued (NBUF - 2 at most) are
+            // outstanding
+            if (G::NBUF == 3 && c + 1 < nch_mine) {
+                if (my_loads == G::LOADS_PER_WAVE) __builtin_amdgcn_s_waitcnt(m4_wait_vm(G::LOADS_PER_WAVE));
+                else __builtin_amdgcn_s_waitcnt(m4_wait_vm(G::LOADS / 2));
             } else {
                 __builtin_amdgcn_s_waitcnt(m4_wait_vm(0));
             }
             __syncthreads();
-            if (c + 2 < nch_mine)
-                m4_issue_buf(rsrc, roff, (c + 2) * M4_KW, bufs + ((c + 2) % M4_NBUF) * M4_BUF_WORDS, xs);
+            if (c + G::NBUF - 1 < nch_mine)
+                m4_issue_buf<KW>(rsrc, roff, (c + G::NBUF - 1) * KW,
+                                 bufs + ((c + G::NBUF - 1) % G::NBUF) * G::BUF_WORDS, xs);
             if (c < nch_mine) {
-                const uint32_t *cur = bufs + (c % M4_NBUF) * M4_BUF_WORDS;
+                const uint32_t *cur = bufs + (c % G::NBUF) * G::BUF_WORDS;
 #pragma unroll
-                for (int s = 0; s < M4_KW / 8; s++) {
+                for (int s = 0; s < KW / 8; s++) {
                     const int q = 2 * s + h;
-                    const uint4 xw = *(const uint4 *)&cur[m3_at(xr, q)];
+                    const uint4 xw = *(const uint4 *)&cur[G::at(xr, q)];
                     uint4 yw[4], zw[2];
 #pragma unroll
-                    for (int rb = 0; rb < 4; rb++) yw[rb] = *(const uint4 *)&cur[m3_at(yr[rb], q)];
-                    zw[0] = *(const uint4 *)&cur[m3_at(zr0, q)];
-                    zw[1] = *(const uint4 *)&cur[m3_at(zr1, q)];
+                    for (int rb = 0; rb < 4; rb++) yw[rb] = *(const uint4 *)&cur[G::at(yr[rb], q)];
+                    zw[0] = *(const uint4 *)&cur[G::at(zr0, q)];
+                    zw[1] = *(const uint4 *)&cur[G::at(zr1, q)];
                     m3_step<0>(xw, yw, zw, acc);
                     m3_step<1>(xw, yw, zw, acc);
                     m3_step<2>(xw, yw, zw, acc);

@@ -274,19 +274,21 @@ int dispatch_tri(episdet_dataset *ds, int tile, TriArgs &a, int grid_override, h
 
 // MFMA tiles: 0 = version 2 (64 x 64 units, one 512-thread workgroup per
 // CU), 1 = version 1 (32 x 64 units, register-staged chunks), 2 = version 3
-// (64 x 32 units, two 256-thread workgroups per CU), 3 = version 4 (pairs of
-// X against 32 x 32, two workgroups per CU, triple-buffered chunks).
-constexpr int MFMA_TILES = 4;
-inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : (tile == 2 ? M3_TY : (tile == 3 ? M4_TY : M2_TY)); }
-inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : (tile == 2 ? M3_TZ : (tile == 3 ? M4_TZ : M2_TZ)); }
-inline int mfma_xstep(int tile) { return tile == 3 ? 2 : 1; }   // X values per work unit
+// (64 x 32 units, two 256-thread workgroups per CU), 3 and 4 = version 4
+// (pairs of X against 32 x 32, two workgroups per CU; 512-sample chunks
+// triple-buffered, or 1024-sample chunks double-buffered).
+constexpr int MFMA_TILES = 5;
+inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : (tile == 2 ? M3_TY : (tile >= 3 ? M4_TY : M2_TY)); }
+inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : (tile == 2 ? M3_TZ : (tile >= 3 ? M4_TZ : M2_TZ)); }
+inline int mfma_xstep(int tile) { return tile >= 3 ? 2 : 1; }   // X values per work unit
 inline int mfma_threads(int tile) { return tile >= 2 ? 256 : 512; }
 template <int MODE>
 const void *mfma_kernel(int tile)
 {
     if (tile == 1) return (const void *)k_triplets_mfma<MODE>;
     if (tile == 2) return (const void *)k_triplets_mfma3<MODE>;
-    if (tile == 3) return (const void *)k_triplets_mfma4<MODE>;
+    if (tile == 3) return (const void *)k_triplets_mfma4<MODE, 16>;
+    if (tile == 4) return (const void *)k_triplets_mfma4<MODE, 32>;
     return (const void *)k_triplets_mfma2<MODE>;
 }
 template <int MODE>
@@ -294,7 +296,8 @@ void launch_tri_mfma(int tile, int blocks, const BatchTri &bt, hipStream_t s)
 {
     if (tile == 1) hipLaunchKernelGGL((k_triplets_mfma<MODE>), dim3(blocks), dim3(512), 0, s, bt);
     else if (tile == 2) hipLaunchKernelGGL((k_triplets_mfma3<MODE>), dim3(blocks), dim3(256), 0, s, bt);
-    else if (tile == 3) hipLaunchKernelGGL((k_triplets_mfma4<MODE>), dim3(blocks), dim3(256), 0, s, bt);
+    else if (tile == 3) hipLaunchKernelGGL((k_triplets_mfma4<MODE, 16>), dim3(blocks), dim3(256), 0, s, bt);
+    else if (tile == 4) hipLaunchKernelGGL((k_triplets_mfma4<MODE, 32>), dim3(blocks), dim3(256), 0, s, bt);
     else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(blocks), dim3(512), 0, s, bt);
 }
 
@@ -924,10 +927,11 @@ int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *til
         return EPISDET_OK;
     }
     *path = mfma_ok(ds) ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
-    // 3-way: version 4 (pairs of X, two workgroups per CU) measured fastest
-    // at both 4096 x 262144 (7.01 s against 7.27 s for version 2) and
-    // 8192 x 65536 (16.8 s against 17.0 s for version 3, 18.4 s for 2).
-    *tile = (*path == EPISDET_PATH_FUSED_MFMA && order == 3) ? 3 : 0;
+    // 3-way: version 4 (pairs of X, two workgroups per CU) with 1024-sample
+    // chunks measured fastest at both 4096 x 262144 (6.79 s against 7.08 s
+    // with 512-sample chunks and 7.27 s for version 2) and 8192 x 65536
+    // (16.2 s against 16.8 s, 17.0 s for version 3, 18.4 s for version 2).
+    *tile = (*path == EPISDET_PATH_FUSED_MFMA && order == 3) ? 4 : 0;
     return EPISDET_OK;
 }
 
