@@ -25,6 +25,17 @@ struct ScoreCtx {
 // controls c1}; genotype 2 is N - c0 - c1 (helper.cu:38-49).
 typedef uint4 Singles;
 
+// Load through a pointer to global memory.  Pointers read from argument
+// blocks in device memory are generic to the compiler, and a generic (flat)
+// load also counts against the LDS wait counter, so an epilogue that mixes
+// them with LDS reads waits on both.
+__device__ __forceinline__ uint4 ld_global(const uint4 *p, size_t i)
+{
+    typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+    const u32x4_t v = ((const __attribute__((address_space(1))) u32x4_t *)p)[i];
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // 3x3 table of the pair (a, b) from its four AND-counts n_ij = |a_i & b_j|,
 // i, j in {0,1}, and the single counts: the five derived cells of
 // helper.cu:644-662.  t[i*3+j].

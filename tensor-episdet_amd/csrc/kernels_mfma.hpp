@@ -241,7 +241,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
                     smem[((((cls * 4 + wih) * 4 + rb * 2 + cb) * 16) + r) * 64 + lane] = (uint32_t)acc[rb][cb][r];
         __syncthreads();
 
-        const Singles sx = p.singles[X];
+        const Singles sx = ld_global(p.singles, X);
         float bad[4];
         unsigned long long packed[4];
         unsigned long long hsum = 0;
@@ -266,7 +266,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
                     n8c[ij * 2 + kk] = smem[((0 * 4 + tw) * 4) * 16 * 64 + off];
                     n8t[ij * 2 + kk] = smem[((1 * 4 + tw) * 4) * 16 * 64 + off];
                 }
-            const Singles sy = p.singles[Y];
+            const Singles sy = ld_global(p.singles, Y);
             uint32_t cas[27], ctl[27];
             triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
             if constexpr (scores_sets(MODE)) {
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
             __syncthreads();
         }
 
-        const Singles sx = p.singles[X];
+        const Singles sx = ld_global(p.singles, X);
         float bad[8];
         unsigned long long packed[8];
         unsigned long long hsum = 0;
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
                         n8c[ij * 2 + kk] = smem[(0 * 4 + twy) * 4 * 16 * 64 + off];
                         n8t[ij * 2 + kk] = smem[(1 * 4 + twy) * 4 * 16 * 64 + off];
                     }
-                const Singles sy = p.singles[Y];
+                const Singles sy = ld_global(p.singles, Y);
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma3(BatchTri bt)
             __syncthreads();
         }
 
-        const Singles sx = p.singles[X];
+        const Singles sx = ld_global(p.singles, X);
         float bad[8];
         unsigned long long packed[8];
         unsigned long long hsum = 0;
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma3(BatchTri bt)
                         n8c[ij * 2 + kk] = smem[(0 * 2 + twy) * 4 * 16 * 64 + off];
                         n8t[ij * 2 + kk] = smem[(1 * 2 + twy) * 4 * 16 * 64 + off];
                     }
-                const Singles sy = p.singles[Y];
+                const Singles sy = ld_global(p.singles, Y);
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
                         n8c[ij * 2 + kk] = smem[(0 * 2 + txs) * 4 * 16 * 64 + off];
                         n8t[ij * 2 + kk] = smem[(1 * 2 + txs) * 4 * 16 * 64 + off];
                     }
-                const Singles sx = p.singles[X], sy = p.singles[Y];
+                const Singles sx = ld_global(p.singles, X), sy = ld_global(p.singles, Y);
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
                     o[0] = make_uint4(n4c[0], n4c[1], n4c[2], n4c[3]);
                     o[1] = make_uint4(n4t[0], n4t[1], n4t[2], n4t[3]);
                 } else {
-                    const Singles sa = p.singles[A], sb = p.singles[B];
+                    const Singles sa = ld_global(p.singles, A), sb = ld_global(p.singles, B);
                     uint32_t cas[9], ctl[9];
                     pair9(n4c[0], n4c[1], n4c[2], n4c[3], sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
                     pair9(n4t[0], n4t[1], n4t[2], n4t[3], sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);

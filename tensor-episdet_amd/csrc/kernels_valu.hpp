@@ -344,7 +344,7 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
                     o[0] = make_uint4(acc0[r][s][0], acc0[r][s][1], acc0[r][s][2], acc0[r][s][3]);
                     o[1] = make_uint4(acc1[r][s][0], acc1[r][s][1], acc1[r][s][2], acc1[r][s][3]);
                 } else {
-                    const Singles sa = p.singles[A], sb = p.singles[B];
+                    const Singles sa = ld_global(p.singles, A), sb = ld_global(p.singles, B);
                     uint32_t cas[9], ctl[9];
                     pair9(acc0[r][s][0], acc0[r][s][1], acc0[r][s][2], acc0[r][s][3], sa.x, sa.y, sb.x, sb.y,
                           (uint32_t)p.N0, cas);
@@ -498,7 +498,7 @@ __device__ __forceinline__ void tri_count_class(const uint32_t *__restrict__ bas
 __device__ __forceinline__ void pair9_lookup(const uint4 *pairs, int n, const Singles &sa, const Singles &sb, int a,
                                              int b, int cls, uint32_t N, uint32_t t[9])
 {
-    const uint4 q = pairs[((size_t)a * n + b) * 2 + cls];
+    const uint4 q = ld_global(pairs, ((size_t)a * n + b) * 2 + cls);
     if (cls == 0) pair9(q.x, q.y, q.z, q.w, sa.x, sa.y, sb.x, sb.y, N, t);
     else pair9(q.x, q.y, q.z, q.w, sa.z, sa.w, sb.z, sb.w, N, t);
 }
@@ -508,7 +508,7 @@ __device__ __forceinline__ void triplet_tables(const TriArgs &p, int X, int Y, i
                                                const Singles &sy, const uint32_t n8c[8], const uint32_t n8t[8],
                                                uint32_t cas[27], uint32_t ctl[27])
 {
-    const Singles sz = p.singles[Z];
+    const Singles sz = ld_global(p.singles, Z);
     uint32_t xy[9], xz[9], yz[9];
     pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 0, (uint32_t)p.N0, xy);
     pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 0, (uint32_t)p.N0, xz);
@@ -569,7 +569,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
                     o1[1] = make_uint4(acc1[a][b][4], acc1[a][b][5], acc1[a][b][6], acc1[a][b][7]);
                 }
         } else {
-            const Singles sx = p.singles[X];
+            const Singles sx = ld_global(p.singles, X);
             float bad[A * B];
             unsigned long long packed[A * B];
             unsigned long long hsum = 0;
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
             for (int a = 0; a < A; a++) {
                 const int Y = y0 + ty + 16 * a;
                 const bool vy = X < Y && Y < p.n;
-                const Singles sy = vy ? p.singles[Y] : Singles{0, 0, 0, 0};
+                const Singles sy = vy ? ld_global(p.singles, Y) : Singles{0, 0, 0, 0};
 #pragma unroll
                 for (int b = 0; b < B; b++) {
                     const int Z = z0 + tz + 16 * b;
@@ -644,7 +644,7 @@ __global__ __launch_bounds__(256) void k_triplets_objective_plain(PlainObjArgs p
             n8c[i] = c0[i];
             n8t[i] = c1[i];
         }
-        const Singles sx = p.singles[X], sy = p.singles[Y], sz = p.singles[Z];
+        const Singles sx = ld_global(p.singles, X), sy = ld_global(p.singles, Y), sz = ld_global(p.singles, Z);
         uint32_t xy[9], xz[9], yz[9], cas[27], ctl[27];
         pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 0, (uint32_t)p.N0, xy);
         pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 0, (uint32_t)p.N0, xz);
@@ -742,7 +742,7 @@ __device__ __forceinline__ void rescore_one(const RescoreArgs &p, int i)
     int nc;
     if (p.order == 3) {
         nc = 27;
-        const Singles sx = p.singles[s[0]], sy = p.singles[s[1]], sz = p.singles[s[2]];
+        const Singles sx = ld_global(p.singles, s[0]), sy = ld_global(p.singles, s[1]), sz = ld_global(p.singles, s[2]);
         uint32_t xy[9], xz[9], yz[9];
         pair9_lookup(p.pairs, p.n, sx, sy, s[0], s[1], 0, (uint32_t)p.N0, xy);
         pair9_lookup(p.pairs, p.n, sx, sz, s[0], s[2], 0, (uint32_t)p.N0, xz);
@@ -754,7 +754,7 @@ __device__ __forceinline__ void rescore_one(const RescoreArgs &p, int i)
         triplet27(n8[1], xy, xz, yz, ctl);
     } else {
         nc = 9;
-        const Singles sa = p.singles[s[0]], sb = p.singles[s[1]];
+        const Singles sa = ld_global(p.singles, s[0]), sb = ld_global(p.singles, s[1]);
         pair9(n8[0][0], n8[0][1], n8[0][2], n8[0][3], sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
         pair9(n8[1][0], n8[1][1], n8[1][2], n8[1][3], sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);
     }

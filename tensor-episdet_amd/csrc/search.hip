@@ -26,6 +26,7 @@
 #include "kernels_mfma_pairs.hpp"
 #include "kernels_mfma3.hpp"
 #include "kernels_mfma4.hpp"
+#include "kernels_mfma_pairs2.hpp"
 
 using namespace episdet;
 
@@ -360,14 +361,30 @@ int launch_pairs_batch(int R, int cu, const BatchPair &bp, hipStream_t s)
     return EPISDET_OK;
 }
 
+// MFMA pair tiles: 0 = 128 x 64 units, one 512-thread workgroup per CU (also
+// the pair-table builder); 1 = 64 x 64 units, two 256-thread workgroups per
+// CU.
+constexpr int MFMA_PAIR_TILES = 2;
 template <int MODE>
-int launch_pairs_mfma(int cu, const BatchPair &bp, hipStream_t s)
+const void *pairs_mfma_kernel(int tile)
 {
-    int per = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_pairs_mfma<MODE>, 512, 0) != hipSuccess ||
-        per < 1)
-        per = 1;
-    hipLaunchKernelGGL((k_pairs_mfma<MODE>), dim3(per * cu), dim3(512), 0, s, bp);
+    if constexpr (MODE != MODE_PAIRTABLE)
+        if (tile == 1) return (const void *)k_pairs_mfma2<MODE>;
+    return (const void *)k_pairs_mfma<MODE>;
+}
+inline int pairs_mfma_threads(int tile) { return tile == 1 ? 256 : 512; }
+
+template <int MODE>
+int launch_pairs_mfma(int tile, int blocks, const BatchPair &bp, hipStream_t s)
+{
+    if constexpr (MODE != MODE_PAIRTABLE) {
+        if (tile == 1) {
+            hipLaunchKernelGGL((k_pairs_mfma2<MODE>), dim3(blocks), dim3(256), 0, s, bp);
+            HIPCHK(hipGetLastError());
+            return EPISDET_OK;
+        }
+    }
+    hipLaunchKernelGGL((k_pairs_mfma<MODE>), dim3(blocks), dim3(512), 0, s, bp);
     HIPCHK(hipGetLastError());
     return EPISDET_OK;
 }
@@ -386,7 +403,14 @@ int dispatch_pairs(episdet_dataset *ds, bool mfma, int tile, PairArgs &a, hipStr
     bp.njobs = 1;
     bp.total = a.nunits;
     bp.queue = a.queue;
-    if (mfma) return launch_pairs_mfma<MODE>(ds->cu, bp, s);
+    if (mfma) {
+        int per = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pairs_mfma_kernel<MODE>(tile), pairs_mfma_threads(tile),
+                                                         0) != hipSuccess ||
+            per < 1)
+            per = 1;
+        return launch_pairs_mfma<MODE>(tile, per * ds->cu, bp, s);
+    }
     return launch_pairs_batch<MODE>(PAIR_TILES[tile], ds->cu, bp, s);
 }
 
@@ -463,7 +487,9 @@ void fill_pair_common(episdet_dataset *ds, PairArgs &a, int T)
 
 void fill_pairs(episdet_dataset *ds, PairArgs &a, bool mfma, int tile)
 {
-    if (mfma) {
+    if (mfma && tile == 1) {
+        fill_pair_common(ds, a, P3_TB);     // 64 x 64 tiles, triangular order
+    } else if (mfma) {
         fill_pair_common(ds, a, P2_TB);
         a.nunits = p2_units(a.ntiles);
     } else {
@@ -651,7 +677,7 @@ struct FusedPlan {
 bool tile_valid(int order, int path, int tile)
 {
     if (tile < 0) return false;
-    if (path == EPISDET_PATH_FUSED_MFMA) return tile < (order == 3 ? MFMA_TILES : 1);
+    if (path == EPISDET_PATH_FUSED_MFMA) return tile < (order == 3 ? MFMA_TILES : MFMA_PAIR_TILES);
     if (order == 3) return tile < (int)(sizeof TRI_TILES / sizeof TRI_TILES[0]);
     return tile < (int)(sizeof PAIR_TILES / sizeof PAIR_TILES[0]);
 }
@@ -811,11 +837,12 @@ unsigned long long count_sets(const episdet_dataset *ds, const FusedPlan &pl, un
                 }
         }
     } else {
-        const long long TA = pl.mfma ? P2_TA : 16 * PAIR_TILES[pl.tile];
-        const long long TB = pl.mfma ? P2_TB : 16 * PAIR_TILES[pl.tile];
+        const bool p2 = pl.mfma && pl.tile == 0;   // 128 x 64 order; the others are triangular
+        const long long TA = p2 ? P2_TA : (pl.mfma ? P3_TA : 16 * PAIR_TILES[pl.tile]);
+        const long long TB = p2 ? P2_TB : (pl.mfma ? P3_TB : 16 * PAIR_TILES[pl.tile]);
         for (unsigned long long u = u0; u < u1; u++) {
             long long ta, tb;
-            if (pl.mfma) host_unrank_p2(u, ta, tb);
+            if (p2) host_unrank_p2(u, ta, tb);
             else host_unrank_tri(u, ta, tb);
             const long long b0 = tb * TB, bhi = std::min(b0 + TB, n);
             for (long long A = ta * TA; A < std::min(ta * TA + TA, n); A++) {
@@ -846,7 +873,9 @@ unsigned long long resident_blocks(const episdet_dataset *ds, const FusedPlan &p
         if (t.A == 4 && t.B == 2) return (unsigned long long)launch_blocks((const void *)k_triplets_valu<4, 2, MODE_SEARCH>, cu);
         return (unsigned long long)launch_blocks((const void *)k_triplets_valu<2, 4, MODE_SEARCH>, cu);
     }
-    if (pl.mfma) return (unsigned long long)occupancy_blocks((const void *)k_pairs_mfma<MODE_SEARCH>, 512, cu);
+    if (pl.mfma)
+        return (unsigned long long)occupancy_blocks(pairs_mfma_kernel<MODE_SEARCH>(pl.tile), pairs_mfma_threads(pl.tile),
+                                                    cu);
     if (PAIR_TILES[pl.tile] == 2) return (unsigned long long)launch_blocks((const void *)k_pairs_valu<2, MODE_SEARCH>, cu);
     return (unsigned long long)launch_blocks((const void *)k_pairs_valu<4, MODE_SEARCH>, cu);
 }
@@ -859,8 +888,8 @@ int tune_kernel(episdet_dataset *ds, int order, int score, std::vector<episdet_t
     std::vector<std::pair<int, int>> cands;
     if (mfma_ok(ds)) {
         cands.push_back({EPISDET_PATH_FUSED_MFMA, 0});
-        if (order == 3)
-            for (int t = 1; t < MFMA_TILES; t++) cands.push_back({EPISDET_PATH_FUSED_MFMA, t});
+        for (int t = 1; t < (order == 3 ? MFMA_TILES : MFMA_PAIR_TILES); t++)
+            cands.push_back({EPISDET_PATH_FUSED_MFMA, t});
     }
     if (order == 3) {
         cands.push_back({EPISDET_PATH_FUSED_VALU, 0});
@@ -931,7 +960,9 @@ int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *til
     // chunks measured fastest at both 4096 x 262144 (6.79 s against 7.08 s
     // with 512-sample chunks and 7.27 s for version 2) and 8192 x 65536
     // (16.2 s against 16.8 s, 17.0 s for version 3, 18.4 s for version 2).
-    *tile = (*path == EPISDET_PATH_FUSED_MFMA && order == 3) ? 4 : 0;
+    // 2-way: the 64 x 64 kernel with the classes in sequence (tile 1)
+    // measured 11.9 ms against 12.8 ms at 16384 x 65536 (40/60 classes).
+    *tile = *path == EPISDET_PATH_FUSED_MFMA ? (order == 3 ? 4 : 1) : 0;
     return EPISDET_OK;
 }
 
@@ -1413,7 +1444,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
 {
     if (!dss || !opts || !results || njobs <= 0) return EPISDET_ERR_INVALID;
     const double t0 = now_s();
-    std::vector<int> tri[MFMA_TILES], pairs, solo;   // 3-way jobs by MFMA tile, 2-way jobs, the rest
+    // 3-way jobs by MFMA tile, 2-way jobs by MFMA pair tile, the rest
+    std::vector<int> tri[MFMA_TILES], pairs[MFMA_PAIR_TILES], solo;
     std::vector<const episdet_dataset *> seen;     // datasets already in the graph
     for (int j = 0; j < njobs; j++) {
         episdet_dataset *ds = dss[j];
@@ -1434,23 +1466,22 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         if (o.order == 3) CHK(ensure_pairs(ds));
         CHK(ensure_cand(ds, o.candidate_cap > 0 ? (unsigned)o.candidate_cap : DEFAULT_CAP));
         results[j].prep_ms = (now_s() - tp) * 1e3;
-        // 3-way jobs go to the batched FP4-MFMA kernel of their tile, 2-way
-        // jobs to the batched FP4-MFMA pair kernel; anything else runs on its
-        // own afterwards.
+        // 3-way and 2-way jobs go to the batched FP4-MFMA kernel of their
+        // tile; anything else runs on its own afterwards.
         // A dataset's control block and candidate buffer hold one search at a
         // time, so a second job on the same dataset runs after the graph.
         const bool dup = std::find(seen.begin(), seen.end(), ds) != seen.end();
         int kpath = o.path, ktile = o.tile;
         if (o.path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, o.order, o.score, &kpath, &ktile, o.autotune != 0));
-        const bool batchable =
-            kpath == EPISDET_PATH_FUSED_MFMA && mfma_ok(ds) && (ktile == 0 || (o.order == 3 && ktile < MFMA_TILES));
+        const bool batchable = kpath == EPISDET_PATH_FUSED_MFMA && mfma_ok(ds) && ktile >= 0 &&
+                               ktile < (o.order == 3 ? MFMA_TILES : MFMA_PAIR_TILES);
         if (dup || !batchable) {
             solo.push_back(j);
             continue;
         }
         seen.push_back(ds);
         if (o.order == 3) tri[ktile].push_back(j);
-        else pairs.push_back(j);
+        else pairs[ktile].push_back(j);
     }
     DevList dl;
     int cu = dss[0]->cu;
@@ -1479,17 +1510,21 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             tstart[tl].push_back(tstart[tl].back() + gstart.back());
         }
     }
-    std::vector<PairArgs> pa(pairs.size());
-    std::vector<unsigned long long> pstart(1, 0ull);
-    for (size_t k = 0; k < pairs.size(); k++) {
-        episdet_dataset *ds = dss[pairs[k]];
-        fill_pairs(ds, pa[k], true, 0);
-        pa[k].sc = make_ctx(ds, opts[pairs[k]].score, 2);
-        pstart.push_back(pstart.back() + pa[k].nunits);
+    std::vector<PairArgs> pa[MFMA_PAIR_TILES];
+    std::vector<unsigned long long> pstart[MFMA_PAIR_TILES];
+    for (int tl = 0; tl < MFMA_PAIR_TILES; tl++) {
+        pa[tl].resize(pairs[tl].size());
+        pstart[tl].assign(1, 0ull);
+        for (size_t k = 0; k < pairs[tl].size(); k++) {
+            episdet_dataset *ds = dss[pairs[tl][k]];
+            fill_pairs(ds, pa[tl][k], true, tl);
+            pa[tl][k].sc = make_ctx(ds, opts[pairs[tl][k]].score, 2);
+            pstart[tl].push_back(pstart[tl].back() + pa[tl][k].nunits);
+        }
     }
     std::vector<int> batched;
     for (int tl = 0; tl < MFMA_TILES; tl++) batched.insert(batched.end(), tri[tl].begin(), tri[tl].end());
-    batched.insert(batched.end(), pairs.begin(), pairs.end());
+    for (int tl = 0; tl < MFMA_PAIR_TILES; tl++) batched.insert(batched.end(), pairs[tl].begin(), pairs[tl].end());
     std::vector<RescoreJob> rj(batched.size());
     std::vector<unsigned long long *> qp(batched.size());
     std::vector<unsigned *> bp(batched.size()), cp(batched.size());
@@ -1518,19 +1553,21 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         cp[k] = &ds->ctl->cand_count;
     }
     TriArgs *d_ta[MFMA_TILES] = {};
-    PairArgs *d_pa = nullptr;
-    unsigned long long *d_tstart[MFMA_TILES] = {}, *d_pstart = nullptr, *d_bq = nullptr;
+    PairArgs *d_pa[MFMA_PAIR_TILES] = {};
+    unsigned long long *d_tstart[MFMA_TILES] = {}, *d_pstart[MFMA_PAIR_TILES] = {}, *d_bq = nullptr;
     RescoreJob *d_rj = nullptr;
     unsigned long long **d_qp = nullptr;
     unsigned **d_bp = nullptr, **d_cp = nullptr;
-    constexpr int NQ = MFMA_TILES + 1;   // one work queue per batched kernel
+    constexpr int NQ = MFMA_TILES + MFMA_PAIR_TILES;   // one work queue per batched kernel
     const unsigned long long zeros[NQ] = {};
     for (int tl = 0; tl < MFMA_TILES; tl++) {
         CHK(dl.put(&d_ta[tl], ta[tl].data(), ta[tl].size()));
         CHK(dl.put(&d_tstart[tl], tstart[tl].data(), tstart[tl].size()));
     }
-    CHK(dl.put(&d_pa, pa.data(), pa.size()));
-    CHK(dl.put(&d_pstart, pstart.data(), pstart.size()));
+    for (int tl = 0; tl < MFMA_PAIR_TILES; tl++) {
+        CHK(dl.put(&d_pa[tl], pa[tl].data(), pa[tl].size()));
+        CHK(dl.put(&d_pstart[tl], pstart[tl].data(), pstart[tl].size()));
+    }
     CHK(dl.put(&d_bq, zeros, NQ));
     CHK(dl.put(&d_rj, rj.data(), rj.size()));
     CHK(dl.put(&d_qp, qp.data(), qp.size()));
@@ -1539,7 +1576,6 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
 
     double gms = 0;
     if (!batched.empty()) {
-        const int blocksp = occupancy_blocks((const void *)k_pairs_mfma<MODE_SEARCH>, 512, cu);
         hipStream_t st, br[NQ];
         HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
         for (int q = 0; q < NQ; q++) HIPCHK(hipStreamCreateWithFlags(&br[q], hipStreamNonBlocking));
@@ -1548,8 +1584,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         for (int q = 0; q < NQ; q++) HIPCHK(hipEventCreateWithFlags(&join[q], hipEventDisableTiming));
         hipGraph_t graph;
         hipGraphExec_t exec;
-        // one graph: reset, then one branch per batched kernel (3-way tile 0,
-        // 3-way tile 1, 2-way) running side by side, so each fills the CUs
+        // one graph: reset, then one branch per batched kernel (each 3-way
+        // and 2-way MFMA tile) running side by side, so each fills the CUs
         // the others' tails leave idle, then FP64 re-scoring
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const int nb = (int)batched.size();
@@ -1568,14 +1604,18 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             launch_tri_mfma<MODE_SEARCH>(tl, occupancy_blocks(mfma_kernel<MODE_SEARCH>(tl), mfma_threads(tl), cu), bt,
                                          br[tl]);
         }
-        if (!pairs.empty()) {
+        for (int tl = 0; tl < MFMA_PAIR_TILES; tl++) {
+            if (pairs[tl].empty()) continue;
             BatchPair bpr;
-            bpr.jobs = d_pa;
-            bpr.job_start = d_pstart;
-            bpr.njobs = (int)pairs.size();
-            bpr.total = pstart.back();
-            bpr.queue = d_bq + MFMA_TILES;
-            hipLaunchKernelGGL((k_pairs_mfma<MODE_SEARCH>), dim3(blocksp), dim3(512), 0, br[MFMA_TILES], bpr);
+            bpr.jobs = d_pa[tl];
+            bpr.job_start = d_pstart[tl];
+            bpr.njobs = (int)pairs[tl].size();
+            bpr.total = pstart[tl].back();
+            bpr.queue = d_bq + MFMA_TILES + tl;
+            // a failed launch shows at the end of the capture
+            (void)launch_pairs_mfma<MODE_SEARCH>(
+                tl, occupancy_blocks(pairs_mfma_kernel<MODE_SEARCH>(tl), pairs_mfma_threads(tl), cu), bpr,
+                br[MFMA_TILES + tl]);
         }
         for (int q = 0; q < NQ; q++) {
             HIPCHK(hipEventRecord(join[q], br[q]));

@@ -81,7 +81,7 @@ def test_fused_tables_bit_exact(n, Nc, Nt, wc, wt, seed):
     syn = episdet.generate(n, Nc, Nt, seed=seed, cases_words=wc, controls_words=wt)
     with episdet.Dataset.from_synthetic(syn) as ds:
         V, M = episdet.PATH_FUSED_VALU, episdet.PATH_FUSED_MFMA
-        for order, path, tile in ((2, V, 0), (2, M, 0), (3, V, 0), (3, M, 0), (3, M, 1), (3, M, 2), (3, M, 3),
+        for order, path, tile in ((2, V, 0), (2, M, 0), (2, M, 1), (3, V, 0), (3, M, 0), (3, M, 1), (3, M, 2), (3, M, 3),
                                   (3, M, 4)):
             say(f"dump order={order} path={path} tile={tile} n={n} Nc={Nc} Nt={Nt}")
             dump = ds.dump_tables(order, path=path, tile=tile)
@@ -116,6 +116,16 @@ def test_mfma_versions_agree(tile):
     syn = episdet.generate(170, 2500, 2100, seed=19, planted_sets=1)
     with episdet.Dataset.from_synthetic(syn) as ds:
         check_best(ds, syn.cases, syn.controls, 2500, 2100, 3, episdet.K2, path=episdet.PATH_FUSED_MFMA, tile=tile)
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+def test_mfma_pair_versions_agree(tile):
+    """Both 2-way MFMA kernels on uneven classes (the batch and autotuner
+    choose between them): winner and FP64 score equal to the oracle's."""
+    syn = episdet.generate(700, 2600, 3900, seed=23, planted_sets=1, planted_order=2)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for score in (episdet.K2, episdet.MI):
+            check_best(ds, syn.cases, syn.controls, 2600, 3900, 2, score, path=episdet.PATH_FUSED_MFMA, tile=tile)
 
 
 def test_plain_path_matches_oracle():
@@ -198,14 +208,14 @@ def test_config1_all_pair_tables():
     syn = episdet.generate(512, 1024, 1024, seed=1)
     with episdet.Dataset.from_synthetic(syn) as ds:
         ref, cs = O.all_tables(syn.cases, syn.controls, 1024, 1024, 2)
-        for path in (episdet.PATH_FUSED_VALU, episdet.PATH_FUSED_MFMA):
-            say(f"config1 dump path={path}")
-            dump = ds.dump_tables(2, path=path)
+        for path, tile in ((episdet.PATH_FUSED_VALU, 0), (episdet.PATH_FUSED_MFMA, 0), (episdet.PATH_FUSED_MFMA, 1)):
+            say(f"config1 dump path={path} tile={tile}")
+            dump = ds.dump_tables(2, path=path, tile=tile)
             assert dump.shape[0] == 130_816
             assert (dump == ref).all()
-            assert ds.checksum(2, path=path) == cs
+            assert ds.checksum(2, path=path, tile=tile) == cs
             for score in (episdet.K2, episdet.MI):
-                check_best(ds, syn.cases, syn.controls, 1024, 1024, 2, score, path=path)
+                check_best(ds, syn.cases, syn.controls, 1024, 1024, 2, score, path=path, tile=tile)
 
 
 def test_full_sample_axis_planted_triplet():
@@ -274,7 +284,7 @@ def test_batch_of_uneven_jobs_matches_oracle():
     say("explicit MFMA tiles 0 and 1 side by side in one graph")
     M = episdet.PATH_FUSED_MFMA
     res4, _ = episdet.run_batch(dss, [o for o, _, _ in jobs], [s for _, s, _ in jobs], paths=[M] * len(jobs),
-                                tiles=[(j // 2) % 5 if o == 3 else 0 for j, (o, _, _) in enumerate(jobs)])
+                                tiles=[(j // 2) % 5 if o == 3 else (j // 2) % 2 for j, (o, _, _) in enumerate(jobs)])
     assert [r["packed"] for r in res4] == [r["packed"] for r in res]
     assert [r["best_score"] for r in res4] == [r["best_score"] for r in res]
     say("one dataset in two jobs of one batch")
@@ -292,6 +302,7 @@ TOPK_CASES = [
     # order, path, tile, n, Nc, Nt, seed
     (2, episdet.PATH_FUSED_VALU, 0, 150, 900, 1100, 31),
     (2, episdet.PATH_FUSED_MFMA, 0, 200, 1100, 700, 36),
+    (2, episdet.PATH_FUSED_MFMA, 1, 230, 1300, 800, 39),
     (3, episdet.PATH_FUSED_VALU, 0, 45, 1500, 1300, 32),
     (3, episdet.PATH_FUSED_MFMA, 0, 100, 2000, 1800, 33),
     (3, episdet.PATH_FUSED_MFMA, 1, 60, 700, 2100, 34),
@@ -368,7 +379,7 @@ def test_autotune_then_auto_matches_oracle(order, n, Nc, Nt):
     with episdet.Dataset.from_synthetic(syn) as ds:
         say(f"autotune order={order} n={n}")
         table, chosen = ds.autotune(order, episdet.K2)
-        expect = (8 if order == 3 else 3)    # 3-way: MFMA tiles 0-4, VALU tiles 0, 1, 3; 2-way: MFMA, VALU R=4, 2
+        expect = (8 if order == 3 else 4)    # 3-way: MFMA tiles 0-4, VALU tiles 0, 1, 3; 2-way: MFMA 0-1, VALU R=4, 2
         assert len(table) == expect and 0 <= chosen < expect
         assert all(e["sample_ms"] > 0 and e["sample_sets"] > 0 and e["est_ms"] > 0 for e in table)
         assert table[chosen]["est_ms"] == min(e["est_ms"] for e in table)
