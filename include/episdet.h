@@ -92,7 +92,13 @@ typedef struct {
     int order;            /* 2 or 3                                              */
     int score;            /* episdet_score                                       */
     int path;             /* episdet_path                                        */
-    int tile;             /* kernel tile id of the chosen path (ignored by AUTO)  */
+    int tile;             /* kernel tile id of the chosen path (ignored by AUTO):
+                             FUSED_MFMA 3-way 0-4 (0: X x 64 x 64 units; 1: X x
+                             32 x 64; 2: X x 64 x 32; 3 and 4: two X x 32 x 32,
+                             512- or 1024-sample chunks), FUSED_MFMA 2-way 0-1
+                             (0: 128 x 64 units; 1: 64 x 64, classes in
+                             sequence), FUSED_VALU 3-way 0-3, FUSED_VALU 2-way
+                             0-2; PLAIN ignores it                               */
     int plain_rounds;     /* PLAIN only: stop after this many rounds (0 = all);
                              used to time a contiguous slice of the round loop   */
     int candidate_cap;    /* 0 = default (1<<20)                                 */
