@@ -1446,7 +1446,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
     const double t0 = now_s();
     // 3-way jobs by MFMA tile, 2-way jobs by MFMA pair tile, the rest
     std::vector<int> tri[MFMA_TILES], pairs[MFMA_PAIR_TILES], solo;
-    std::vector<const episdet_dataset *> seen;     // datasets already in the graph
+    std::vector<int> kpath(njobs), ktile(njobs);
     for (int j = 0; j < njobs; j++) {
         episdet_dataset *ds = dss[j];
         const episdet_options &o = opts[j];
@@ -1466,22 +1466,49 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         if (o.order == 3) CHK(ensure_pairs(ds));
         CHK(ensure_cand(ds, o.candidate_cap > 0 ? (unsigned)o.candidate_cap : DEFAULT_CAP));
         results[j].prep_ms = (now_s() - tp) * 1e3;
+        kpath[j] = o.path;
+        ktile[j] = o.tile;
+        if (o.path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, o.order, o.score, &kpath[j], &ktile[j], o.autotune != 0));
+    }
+    // AUTO jobs of one order share one kernel: the MFMA tile their per-shape
+    // choices give the most work (sets x samples).  One graph branch per order
+    // packs the CUs better than one branch per tile: on the 64-job workload
+    // the per-job choices, spread over seven kernels, took 1.75 s against
+    // 1.51 s for one 3-way and one 2-way kernel.
+    for (int order = 2; order <= 3; order++) {
+        const int ntiles = order == 3 ? MFMA_TILES : MFMA_PAIR_TILES;
+        std::vector<double> work(ntiles, 0.0);
+        for (int j = 0; j < njobs; j++)
+            if (opts[j].order == order && opts[j].path == EPISDET_PATH_AUTO && kpath[j] == EPISDET_PATH_FUSED_MFMA &&
+                ktile[j] >= 0 && ktile[j] < ntiles)
+                work[ktile[j]] += (double)episdet_n_choose_k((unsigned)dss[j]->n, (unsigned)order) *
+                                  (double)(dss[j]->N[0] + dss[j]->N[1]);
+        const int win = (int)(std::max_element(work.begin(), work.end()) - work.begin());
+        if (work[win] <= 0.0) continue;
+        for (int j = 0; j < njobs; j++)
+            if (opts[j].order == order && opts[j].path == EPISDET_PATH_AUTO && mfma_ok(dss[j])) {
+                kpath[j] = EPISDET_PATH_FUSED_MFMA;
+                ktile[j] = win;
+            }
+    }
+    std::vector<const episdet_dataset *> seen;     // datasets already in the graph
+    for (int j = 0; j < njobs; j++) {
+        episdet_dataset *ds = dss[j];
+        const episdet_options &o = opts[j];
         // 3-way and 2-way jobs go to the batched FP4-MFMA kernel of their
         // tile; anything else runs on its own afterwards.
         // A dataset's control block and candidate buffer hold one search at a
         // time, so a second job on the same dataset runs after the graph.
         const bool dup = std::find(seen.begin(), seen.end(), ds) != seen.end();
-        int kpath = o.path, ktile = o.tile;
-        if (o.path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, o.order, o.score, &kpath, &ktile, o.autotune != 0));
-        const bool batchable = kpath == EPISDET_PATH_FUSED_MFMA && mfma_ok(ds) && ktile >= 0 &&
-                               ktile < (o.order == 3 ? MFMA_TILES : MFMA_PAIR_TILES);
+        const bool batchable = kpath[j] == EPISDET_PATH_FUSED_MFMA && mfma_ok(ds) && ktile[j] >= 0 &&
+                               ktile[j] < (o.order == 3 ? MFMA_TILES : MFMA_PAIR_TILES);
         if (dup || !batchable) {
             solo.push_back(j);
             continue;
         }
         seen.push_back(ds);
-        if (o.order == 3) tri[ktile].push_back(j);
-        else pairs[ktile].push_back(j);
+        if (o.order == 3) tri[ktile[j]].push_back(j);
+        else pairs[ktile[j]].push_back(j);
     }
     DevList dl;
     int cu = dss[0]->cu;
