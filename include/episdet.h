@@ -93,9 +93,10 @@ typedef struct {
     int score;            /* episdet_score                                       */
     int path;             /* episdet_path                                        */
     int tile;             /* kernel tile id of the chosen path (ignored by AUTO):
-                             FUSED_MFMA 3-way 0-4 (0: X x 64 x 64 units; 1: X x
+                             FUSED_MFMA 3-way 0-5 (0: X x 64 x 64 units; 1: X x
                              32 x 64; 2: X x 64 x 32; 3 and 4: two X x 32 x 32,
-                             512- or 1024-sample chunks), FUSED_MFMA 2-way 0-1
+                             512- or 1024-sample chunks; 5: as 4, classes in
+                             sequence), FUSED_MFMA 2-way 0-1
                              (0: 128 x 64 units; 1: 64 x 64, classes in
                              sequence), FUSED_VALU 3-way 0-3, FUSED_VALU 2-way
                              0-2; PLAIN ignores it                               */

@@ -27,6 +27,7 @@
 #include "kernels_mfma3.hpp"
 #include "kernels_mfma4.hpp"
 #include "kernels_mfma_pairs2.hpp"
+#include "kernels_mfma5.hpp"
 
 using namespace episdet;
 
@@ -277,8 +278,9 @@ int dispatch_tri(episdet_dataset *ds, int tile, TriArgs &a, int grid_override, h
 // CU), 1 = version 1 (32 x 64 units, register-staged chunks), 2 = version 3
 // (64 x 32 units, two 256-thread workgroups per CU), 3 and 4 = version 4
 // (pairs of X against 32 x 32, two workgroups per CU; 512-sample chunks
-// triple-buffered, or 1024-sample chunks double-buffered).
-constexpr int MFMA_TILES = 5;
+// triple-buffered, or 1024-sample chunks double-buffered), 5 = version 5
+// (version 4's unit with the classes contracted in sequence by all waves).
+constexpr int MFMA_TILES = 6;
 inline int mfma_ty(int tile) { return tile == 1 ? MF_TY : (tile == 2 ? M3_TY : (tile >= 3 ? M4_TY : M2_TY)); }
 inline int mfma_tz(int tile) { return tile == 1 ? MF_TZ : (tile == 2 ? M3_TZ : (tile >= 3 ? M4_TZ : M2_TZ)); }
 inline int mfma_xstep(int tile) { return tile >= 3 ? 2 : 1; }   // X values per work unit
@@ -290,6 +292,7 @@ const void *mfma_kernel(int tile)
     if (tile == 2) return (const void *)k_triplets_mfma3<MODE>;
     if (tile == 3) return (const void *)k_triplets_mfma4<MODE, 16>;
     if (tile == 4) return (const void *)k_triplets_mfma4<MODE, 32>;
+    if (tile == 5) return (const void *)k_triplets_mfma5<MODE>;
     return (const void *)k_triplets_mfma2<MODE>;
 }
 template <int MODE>
@@ -299,6 +302,7 @@ void launch_tri_mfma(int tile, int blocks, const BatchTri &bt, hipStream_t s)
     else if (tile == 2) hipLaunchKernelGGL((k_triplets_mfma3<MODE>), dim3(blocks), dim3(256), 0, s, bt);
     else if (tile == 3) hipLaunchKernelGGL((k_triplets_mfma4<MODE, 16>), dim3(blocks), dim3(256), 0, s, bt);
     else if (tile == 4) hipLaunchKernelGGL((k_triplets_mfma4<MODE, 32>), dim3(blocks), dim3(256), 0, s, bt);
+    else if (tile == 5) hipLaunchKernelGGL((k_triplets_mfma5<MODE>), dim3(blocks), dim3(256), 0, s, bt);
     else hipLaunchKernelGGL((k_triplets_mfma2<MODE>), dim3(blocks), dim3(512), 0, s, bt);
 }
 
@@ -962,7 +966,14 @@ int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *til
     // (16.2 s against 16.8 s, 17.0 s for version 3, 18.4 s for version 2).
     // 2-way: the 64 x 64 kernel with the classes in sequence (tile 1)
     // measured 11.9 ms against 12.8 ms at 16384 x 65536 (40/60 classes).
-    *tile = *path == EPISDET_PATH_FUSED_MFMA ? (order == 3 ? 4 : 1) : 0;
+    // 3-way with uneven classes: version 5 (tile 5) takes the same time for
+    // any split, version 4 is 12% faster at 50/50 but slower once the
+    // smaller class has under ~2/3 of the larger's samples (2048 SNPs x
+    // 131072 samples: 452 / 517 / 581 / 659 ms for version 4 at 50/50,
+    // 40/60, 30/70, 20/80 against 505 / 512 / 512 / 513 ms for version 5).
+    if (*path != EPISDET_PATH_FUSED_MFMA) *tile = 0;
+    else if (order == 2) *tile = 1;
+    else *tile = 100LL * std::min(ds->W[0], ds->W[1]) < 65LL * std::max(ds->W[0], ds->W[1]) ? 5 : 4;
     return EPISDET_OK;
 }
 

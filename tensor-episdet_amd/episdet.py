@@ -370,7 +370,7 @@ class Dataset:
     def autotune(self, order, score=K2):
         """Times every kernel configuration for this shape and caches the
         fastest for AUTO: (list of dicts, index of the chosen one)."""
-        cap = 8
+        cap = 32                                # more than the candidate kernels of any shape
         arr = (TuneEntry * cap)()
         cnt, chosen = C.c_int(), C.c_int()
         rc = self._lib.episdet_autotune(self._h, order, score, arr, cap, C.byref(cnt), C.byref(chosen))

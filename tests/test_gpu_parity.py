@@ -82,7 +82,7 @@ def test_fused_tables_bit_exact(n, Nc, Nt, wc, wt, seed):
     with episdet.Dataset.from_synthetic(syn) as ds:
         V, M = episdet.PATH_FUSED_VALU, episdet.PATH_FUSED_MFMA
         for order, path, tile in ((2, V, 0), (2, M, 0), (2, M, 1), (3, V, 0), (3, M, 0), (3, M, 1), (3, M, 2), (3, M, 3),
-                                  (3, M, 4)):
+                                  (3, M, 4), (3, M, 5)):
             say(f"dump order={order} path={path} tile={tile} n={n} Nc={Nc} Nt={Nt}")
             dump = ds.dump_tables(order, path=path, tile=tile)
             ref, cs = O.all_tables(syn.cases, syn.controls, Nc, Nt, order)
@@ -111,11 +111,22 @@ def test_triplet_tiles_agree(tile):
         check_best(ds, syn.cases, syn.controls, 1500, 1700, 3, episdet.K2, tile=tile)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 def test_mfma_versions_agree(tile):
     syn = episdet.generate(170, 2500, 2100, seed=19, planted_sets=1)
     with episdet.Dataset.from_synthetic(syn) as ds:
         check_best(ds, syn.cases, syn.controls, 2500, 2100, 3, episdet.K2, path=episdet.PATH_FUSED_MFMA, tile=tile)
+
+
+def test_auto_runs_sequenced_classes_for_uneven_classes():
+    """AUTO runs version 5 (tile 5, classes in sequence) when one class has
+    under 65% of the other's words, version 4 (tile 4) otherwise; both give
+    the oracle's winner."""
+    for Nc, Nt, tile in ((1500, 4600, 5), (3000, 3100, 4)):
+        syn = episdet.generate(90, Nc, Nt, seed=61, planted_sets=1)
+        with episdet.Dataset.from_synthetic(syn) as ds:
+            r = check_best(ds, syn.cases, syn.controls, Nc, Nt, 3, episdet.K2, path=episdet.PATH_AUTO)
+            assert (r["path"], r["tile"]) == (episdet.PATH_FUSED_MFMA, tile)
 
 
 @pytest.mark.parametrize("tile", [0, 1])
@@ -224,13 +235,13 @@ def test_full_sample_axis_planted_triplet():
     syn = episdet.generate(192, 131072, 131072, seed=2, planted_sets=1, planted_order=3)
     with episdet.Dataset.from_synthetic(syn) as ds:
         for path, tile in ((episdet.PATH_FUSED_VALU, 0), (episdet.PATH_FUSED_MFMA, 0), (episdet.PATH_FUSED_MFMA, 2),
-                           (episdet.PATH_FUSED_MFMA, 3), (episdet.PATH_FUSED_MFMA, 4)):
+                           (episdet.PATH_FUSED_MFMA, 3), (episdet.PATH_FUSED_MFMA, 4), (episdet.PATH_FUSED_MFMA, 5)):
             r = check_best(ds, syn.cases, syn.controls, 131072, 131072, 3, episdet.K2, path=path, tile=tile)
             assert tuple(r["idx"]) == syn.planted[0]
             assert r["fp32_max_rel_err"] <= K2_FP32_REL_TOL
         say("checksums at the full sample axis: MFMA tiles vs VALU")
         cv = ds.checksum(3, path=episdet.PATH_FUSED_VALU)
-        for tile in (0, 1, 2, 3, 4):
+        for tile in (0, 1, 2, 3, 4, 5):
             assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA, tile=tile) == cv
         say("checksum at full sample axis vs plain tables")
         sets = np.array(list(itertools.islice(itertools.combinations(range(192), 3), 0, 4000, 7)), dtype=np.int32)
@@ -284,7 +295,7 @@ def test_batch_of_uneven_jobs_matches_oracle():
     say("explicit MFMA tiles 0 and 1 side by side in one graph")
     M = episdet.PATH_FUSED_MFMA
     res4, _ = episdet.run_batch(dss, [o for o, _, _ in jobs], [s for _, s, _ in jobs], paths=[M] * len(jobs),
-                                tiles=[(j // 2) % 5 if o == 3 else (j // 2) % 2 for j, (o, _, _) in enumerate(jobs)])
+                                tiles=[(j // 2) % 6 if o == 3 else (j // 2) % 2 for j, (o, _, _) in enumerate(jobs)])
     assert [r["packed"] for r in res4] == [r["packed"] for r in res]
     assert [r["best_score"] for r in res4] == [r["best_score"] for r in res]
     say("one dataset in two jobs of one batch")
@@ -308,6 +319,7 @@ TOPK_CASES = [
     (3, episdet.PATH_FUSED_MFMA, 1, 60, 700, 2100, 34),
     (3, episdet.PATH_FUSED_MFMA, 3, 70, 1500, 900, 37),
     (3, episdet.PATH_FUSED_MFMA, 4, 80, 2500, 1200, 38),
+    (3, episdet.PATH_FUSED_MFMA, 5, 75, 900, 2300, 40),
 ]
 
 
@@ -379,7 +391,7 @@ def test_autotune_then_auto_matches_oracle(order, n, Nc, Nt):
     with episdet.Dataset.from_synthetic(syn) as ds:
         say(f"autotune order={order} n={n}")
         table, chosen = ds.autotune(order, episdet.K2)
-        expect = (8 if order == 3 else 4)    # 3-way: MFMA tiles 0-4, VALU tiles 0, 1, 3; 2-way: MFMA 0-1, VALU R=4, 2
+        expect = (9 if order == 3 else 4)    # 3-way: MFMA tiles 0-5, VALU tiles 0, 1, 3; 2-way: MFMA 0-1, VALU R=4, 2
         assert len(table) == expect and 0 <= chosen < expect
         assert all(e["sample_ms"] > 0 and e["sample_sets"] > 0 and e["est_ms"] > 0 for e in table)
         assert table[chosen]["est_ms"] == min(e["est_ms"] for e in table)
