@@ -19,6 +19,10 @@ struct ScoreCtx {
     float invN;         // 1/N
     float hY;           // MI: H(Y) in bits
     float margin;       // candidate margin, 2 x the FP32 error bound
+    // K2 lower bound (k2_lower_bound): pi = Nc/N, N H(pi) in nats, the
+    // curvature constant, and the slack a bound must clear before a set's
+    // FP32 score may be skipped (2 margins plus the bound's own FP32 error)
+    float lb_pi, lb_nh, lb_k, lb_slack;
 };
 
 // Single-SNP counts of both classes: {cases c0, cases c1, controls c0,
@@ -167,6 +171,26 @@ __device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl
     }
     // H(G) = -sG, H(G,Y) = -sGY
     return (sGY - sG) + hY;
+}
+
+// Lower bound of the K2 score of a table, from two inequalities:
+//   ln((n+1)!/(a! b!)) = ln(n+1) + ln C(n, a) >= n H(a/n), since
+//     C(n, k) >= e^{n H(k/n)} / (n + 1)  (method of types);
+//   H(p) >= H(pi) + H'(pi) (p - pi) - K (p - pi)^2 on [0, 1], with
+//     K = max(-ln(1-pi)/pi^2, -ln(pi)/(1-pi)^2), the largest KL(p||pi)/(p-pi)^2.
+// Summed over the cells with pi = Nc/N the linear terms cancel:
+//   K2 >= N H(pi) - K sum_c (cas_c - pi n_c)^2 / n_c.
+// Cells with n = 0 contribute 0 to both sides.
+__device__ __forceinline__ float k2_lower_bound(const uint32_t *cas, const uint32_t *ctl, int ncells, float pi,
+                                               float nh, float kq)
+{
+    float s = 0.f;
+    for (int i = 0; i < ncells; i++) {
+        const uint32_t nn = cas[i] + ctl[i];
+        const float n = (float)nn, d = fmaf(-pi, n, (float)cas[i]);
+        s += nn ? d * d * __builtin_amdgcn_rcpf(n) : 0.f;
+    }
+    return fmaf(-kq, s, nh);
 }
 
 // "Badness": the FP32 score mapped so that lower is always better.

@@ -1,0 +1,88 @@
+"""The K2 lower bound with which the 3-way MFMA kernels skip FP32 scores that
+cannot matter (k2_lower_bound in device_common.hpp):
+
+    K2 >= N H(pi) - K sum_c (cas_c - pi n_c)^2 / n_c,   pi = Nc / N,
+    K = max(-ln(1 - pi) / pi^2, -ln(pi) / (1 - pi)^2),
+
+from C(n, k) >= e^{n H(k/n)} / (n + 1) and H(p) >= H(pi) + H'(pi)(p - pi)
+- K (p - pi)^2 on [0, 1].  Checked on the CPU against the oracle's exact
+FP64 K2 over random, skewed and extreme tables; the curvature constant is
+checked to be the supremum it claims, and a halved constant is shown to
+break the bound.
+"""
+import math
+
+import numpy as np
+
+import oracle as O
+
+
+def curvature(pi):
+    return max(-math.log1p(-pi) / pi ** 2, -math.log(pi) / (1 - pi) ** 2)
+
+
+def bound(cas, ctl, k=None):
+    cas = np.asarray(cas, dtype=np.float64)
+    ctl = np.asarray(ctl, dtype=np.float64)
+    Nc, Nt = cas.sum(), ctl.sum()
+    N = Nc + Nt
+    pi = Nc / N
+    if k is None:
+        k = curvature(pi)
+    n = cas + ctl
+    m = n > 0
+    s = (((cas - pi * n) ** 2)[m] / n[m]).sum()
+    nh = -N * (pi * math.log(pi) + (1 - pi) * math.log(1 - pi))
+    return nh - k * s
+
+
+def random_table(rng, cells=27):
+    Nc, Nt = (int(x) for x in rng.integers(1, 6000, 2))
+    kind = rng.integers(0, 4)
+    if kind == 0:                       # same cell distribution in both classes
+        p = rng.dirichlet(np.full(cells, rng.uniform(0.05, 3.0)))
+        q = p
+    elif kind == 1:                     # independent distributions (strong signal)
+        p = rng.dirichlet(np.full(cells, rng.uniform(0.05, 3.0)))
+        q = rng.dirichlet(np.full(cells, rng.uniform(0.05, 3.0)))
+    elif kind == 2:                     # a planted shift of one cell's mass
+        p = rng.dirichlet(np.full(cells, 1.0))
+        q = p.copy()
+        i, j = rng.integers(0, cells, 2)
+        q[i] += 0.3
+        q[j] = max(q[j] - 0.3, 0.0)
+        q /= q.sum()
+    else:                               # few samples, many empty cells
+        Nc, Nt = (int(x) for x in rng.integers(1, 12, 2))
+        p = rng.dirichlet(np.full(cells, 0.2))
+        q = rng.dirichlet(np.full(cells, 0.2))
+    return rng.multinomial(Nc, p), rng.multinomial(Nt, q)
+
+
+def test_curvature_is_the_supremum():
+    for pi in (0.01, 0.1, 0.3, 0.5, 0.62, 0.9, 0.999):
+        p = np.linspace(0.0, 1.0, 200001)
+        p = p[np.abs(p - pi) > 1e-7]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            kl = np.where(p > 0, p * np.log(p / pi), 0.0) + np.where(p < 1, (1 - p) * np.log((1 - p) / (1 - pi)), 0.0)
+        assert (kl / (p - pi) ** 2).max() <= curvature(pi) * (1 + 1e-12)
+
+
+def test_bound_below_exact_k2():
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        cells = 27 if rng.integers(0, 4) else 9
+        cas, ctl = random_table(rng, cells)
+        k2 = O.score_k2(cas, ctl)
+        assert bound(cas, ctl) <= k2 + 1e-9 * max(1.0, abs(k2)), (cas, ctl)
+
+
+def test_bound_is_tight_for_a_separating_table():
+    # every cell pure cases or pure controls, 50/50: the bound is 0 while
+    # K2 = sum ln(n_c + 1); a halved curvature constant breaks it
+    cas = np.array([500] * 13 + [0] * 14, dtype=np.uint32)
+    ctl = np.array([0] * 13 + [464] * 13 + [468], dtype=np.uint32)
+    k2 = O.score_k2(cas, ctl)
+    assert bound(cas, ctl) <= k2
+    pi = cas.sum() / (cas.sum() + ctl.sum())
+    assert bound(cas, ctl, k=curvature(pi) / 2) > k2
