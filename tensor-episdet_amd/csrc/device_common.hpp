@@ -19,10 +19,11 @@ struct ScoreCtx {
     float invN;         // 1/N
     float hY;           // MI: H(Y) in bits
     float margin;       // candidate margin, 2 x the FP32 error bound
-    // K2 lower bound (k2_lower_bound): pi = Nc/N, N H(pi) in nats, the
-    // curvature constant, and the slack a bound must clear before a set's
-    // FP32 score may be skipped (2 margins plus the bound's own FP32 error)
-    float lb_pi, lb_nh, lb_k, lb_slack;
+    // Lower bound of the badness (badness_lower_bound): lb_max - lb_scale * S
+    // with S the table's chi-square sum and pi = Nc/N; lb_slack is what a
+    // bound must clear above the best score before a set's FP32 score may
+    // be skipped (2 margins plus the bound's own FP32 error)
+    float lb_pi, lb_max, lb_scale, lb_slack;
 };
 
 // Single-SNP counts of both classes: {cases c0, cases c1, controls c0,
@@ -173,24 +174,30 @@ __device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl
     return (sGY - sG) + hY;
 }
 
-// Lower bound of the K2 score of a table, from two inequalities:
-//   ln((n+1)!/(a! b!)) = ln(n+1) + ln C(n, a) >= n H(a/n), since
-//     C(n, k) >= e^{n H(k/n)} / (n + 1)  (method of types);
-//   H(p) >= H(pi) + H'(pi) (p - pi) - K (p - pi)^2 on [0, 1], with
-//     K = max(-ln(1-pi)/pi^2, -ln(pi)/(1-pi)^2), the largest KL(p||pi)/(p-pi)^2.
-// Summed over the cells with pi = Nc/N the linear terms cancel:
-//   K2 >= N H(pi) - K sum_c (cas_c - pi n_c)^2 / n_c.
-// Cells with n = 0 contribute 0 to both sides.
-__device__ __forceinline__ float k2_lower_bound(const uint32_t *cas, const uint32_t *ctl, int ncells, float pi,
-                                               float nh, float kq)
+// Lower bound of a table's badness (K2, or -MI), from
+//   KL(p || pi) <= K (p - pi)^2 on [0, 1],  K = max(-ln(1-pi)/pi^2, -ln(pi)/(1-pi)^2)
+// (the largest KL(p||pi)/(p-pi)^2, reached at p = 0 or 1) and the chi-square
+// sum S = sum_c (cas_c - pi n_c)^2 / n_c, pi = Nc/N:
+//  * K2: ln((n+1)!/(a! b!)) = ln(n+1) + ln C(n, a) >= n H(a/n) since
+//    C(n, k) >= e^{n H(k/n)} / (n + 1) (method of types), and
+//    n H(a/n) = n H(pi) + (a - pi n) H'(pi) - n KL(a/n || pi); summed over the
+//    cells the linear terms cancel:  K2 >= N H(pi) - K S  (nats).
+//  * MI: I(G;Y) = sum_c (n_c/N) KL(p_c || pi) <= K S / N nats, so
+//    -MI >= -K S / (N ln 2) in bits.
+// Both are lb_max - lb_scale * S.  Cells with n = 0 add nothing.  NC is
+// the cell count (27 or 9, equal to sc.ncells), fixed at compile time so
+// the short loop unrolls.
+template <int NC>
+__device__ __forceinline__ float badness_lower_bound(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl)
 {
     float s = 0.f;
-    for (int i = 0; i < ncells; i++) {
+#pragma unroll
+    for (int i = 0; i < NC; i++) {
         const uint32_t nn = cas[i] + ctl[i];
-        const float n = (float)nn, d = fmaf(-pi, n, (float)cas[i]);
+        const float n = (float)nn, d = fmaf(-sc.lb_pi, n, (float)cas[i]);
         s += nn ? d * d * __builtin_amdgcn_rcpf(n) : 0.f;
     }
-    return fmaf(-kq, s, nh);
+    return fmaf(-sc.lb_scale, s, sc.lb_max);
 }
 
 // "Badness": the FP32 score mapped so that lower is always better.
@@ -266,6 +273,17 @@ __device__ __forceinline__ float key2f(unsigned k)
     unsigned u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
     return __uint_as_float(u);
 }
+
+// The badness above which a set's FP32 score may be skipped: the best
+// FP32 badness published so far (the global best only decreases, so a
+// stale read only skips less) plus the slack, or +inf while no bound can
+// exceed it (bounds never exceed lb_max).
+__device__ __forceinline__ float skip_threshold(const ScoreCtx &sc, const unsigned *best_key)
+{
+    const float t = key2f(__atomic_load_n(best_key, __ATOMIC_RELAXED)) + sc.lb_slack;
+    return t < sc.lb_max ? t : INFINITY;
+}
+
 __device__ __forceinline__ unsigned long long d2key(double d)
 {
     unsigned long long u = (unsigned long long)__double_as_longlong(d);

@@ -190,18 +190,11 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
         float bad[8];
         unsigned long long packed[8];
         unsigned long long hsum = 0;
-        // Sets whose K2 lower bound exceeds the best FP32 score found so far
-        // by more than the slack cannot become candidates; their FP32 score is
-        // skipped.  The global best only decreases, so a stale read only
-        // skips less.  Bounds stay below N H(pi), so nothing is skipped until
-        // a set scores below that.
+        // Sets whose badness lower bound exceeds the best FP32 score found so
+        // far by more than the slack cannot become candidates; their FP32
+        // score is skipped (skip_threshold).
         float skip_above = INFINITY;
-        if constexpr (MODE == MODE_SEARCH) {
-            if (p.sc.score == SCORE_K2) {
-                const float t = key2f(__atomic_load_n(p.co.best_key, __ATOMIC_RELAXED)) + p.sc.lb_slack;
-                if (t < p.sc.lb_nh) skip_above = t;
-            }
-        }
+        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int hf = 0; hf < 2; hf++) {
             // column block hf of every row block -> LDS [cls][xs][rb][reg][lane]
@@ -240,8 +233,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {
-                    if (skip_above < INFINITY &&
-                        k2_lower_bound(cas, ctl, 27, p.sc.lb_pi, p.sc.lb_nh, p.sc.lb_k) > skip_above)
+                    if (skip_above < INFINITY && badness_lower_bound<27>(p.sc, cas, ctl) > skip_above)
                         continue;                           // bad[m] stays INFINITY
                     bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                 } else if constexpr (MODE == MODE_DUMP) {

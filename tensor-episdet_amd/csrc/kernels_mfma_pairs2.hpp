@@ -170,6 +170,10 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma2(BatchPair bp)
         float bad[16];
         unsigned long long packed[16];
         unsigned long long hsum = 0;
+        // sets whose badness lower bound is above this cannot become
+        // candidates; their FP32 score is skipped (skip_threshold)
+        float skip_above = INFINITY;
+        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int hf = 0; hf < 2; hf++) {
             // column tile hf of every wave -> LDS [class][wave][row tile][reg][lane]
@@ -215,6 +219,8 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma2(BatchPair bp)
                 pair9(n4c[0], n4c[1], n4c[2], n4c[3], sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
                 pair9(n4t[0], n4t[1], n4t[2], n4t[3], sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);
                 if constexpr (scores_sets(MODE)) {
+                    if (skip_above < INFINITY && badness_lower_bound<9>(p.sc, cas, ctl) > skip_above)
+                        continue;                           // bad[m] stays INFINITY
                     bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                 } else if constexpr (MODE == MODE_DUMP) {
                     uint32_t *o = p.dump + rank2(A, B) * 18;

@@ -235,14 +235,26 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
         eps = 2.0 * t.term_err + u * (3.0 * ncells * 0.531 * 2.0 + 8.0 * hmax) + 4.0 * u;
     }
     sc.margin = (float)(2.0 * eps);
-    // K2 lower bound (k2_lower_bound).  Its FP32 error: N H(pi) rounds by
-    // under 1e-7 N; each (cas - pi n)^2 / n errs by at most about
-    // 2 |d| ulp(N) / n <= 2 ulp(N) since |d| <= n, K times that over the
-    // cells; a further 1 covers the sum's roundings.
+    // Badness lower bound (badness_lower_bound): lb_max - lb_scale * S.
+    // Its FP32 error: each (cas - pi n)^2 / n errs by about 2 |d| ulp(N) / n
+    // <= 2 ulp(N) since |d| <= n (taken 4 times over), lb_scale times that
+    // over the cells; K2's N H(pi) rounds by under 1e-7 N, and a further
+    // absolute term covers the sum's roundings.
+    const double K = std::max(-std::log1p(-qc) / (qc * qc), -std::log(qc) / (qt * qt)) * (1.0 + 1e-6);
     sc.lb_pi = (float)qc;
-    sc.lb_k = (float)(std::max(-std::log1p(-qc) / (qc * qc), -std::log(qc) / (qt * qt)) * (1.0 + 1e-6));
-    sc.lb_nh = (float)(-(double)N * (qc * std::log(qc) + qt * std::log(qt)));
-    sc.lb_slack = (float)(2.0 * sc.margin + 1e-7 * N + sc.lb_k * ncells * 2.0 * ulp32((double)N) * 4.0 + 1.0);
+    double scale, lbmax, extra;
+    if (score == EPISDET_SCORE_K2) {
+        scale = K;
+        lbmax = -(double)N * (qc * std::log(qc) + qt * std::log(qt));
+        extra = 1e-7 * N + 1.0;
+    } else {
+        scale = K / ((double)N * std::log(2.0));
+        lbmax = 0.0;
+        extra = 1e-6;
+    }
+    sc.lb_scale = (float)scale;
+    sc.lb_max = (float)lbmax;
+    sc.lb_slack = (float)(2.0 * sc.margin + scale * ncells * 8.0 * ulp32((double)N) + extra);
     return sc;
 }
 

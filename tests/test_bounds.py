@@ -1,14 +1,14 @@
-"""The K2 lower bound with which the 3-way MFMA kernels skip FP32 scores that
-cannot matter (k2_lower_bound in device_common.hpp):
+"""The score bounds with which the MFMA kernels skip FP32 scores that cannot
+matter (badness_lower_bound in device_common.hpp).  With pi = Nc / N,
+S = sum_c (cas_c - pi n_c)^2 / n_c and K = max(-ln(1 - pi) / pi^2,
+-ln(pi) / (1 - pi)^2), the largest KL(p || pi) / (p - pi)^2 on [0, 1]:
 
-    K2 >= N H(pi) - K sum_c (cas_c - pi n_c)^2 / n_c,   pi = Nc / N,
-    K = max(-ln(1 - pi) / pi^2, -ln(pi) / (1 - pi)^2),
+    K2 >= N H(pi) - K S          (nats; from C(n, k) >= e^{n H(k/n)} / (n + 1))
+    MI <= K S / (N ln 2)         (bits; MI = sum_c (n_c / N) KL(p_c || pi))
 
-from C(n, k) >= e^{n H(k/n)} / (n + 1) and H(p) >= H(pi) + H'(pi)(p - pi)
-- K (p - pi)^2 on [0, 1].  Checked on the CPU against the oracle's exact
-FP64 K2 over random, skewed and extreme tables; the curvature constant is
-checked to be the supremum it claims, and a halved constant is shown to
-break the bound.
+Checked on the CPU against the oracle's exact FP64 K2 and MI over random,
+skewed and extreme tables; the curvature constant is checked to be the
+supremum it claims, and a halved constant is shown to break the bounds.
 """
 import math
 
@@ -19,6 +19,23 @@ import oracle as O
 
 def curvature(pi):
     return max(-math.log1p(-pi) / pi ** 2, -math.log(pi) / (1 - pi) ** 2)
+
+
+def chi_sum(cas, ctl):
+    cas = np.asarray(cas, dtype=np.float64)
+    ctl = np.asarray(ctl, dtype=np.float64)
+    pi = cas.sum() / (cas.sum() + ctl.sum())
+    n = cas + ctl
+    m = n > 0
+    return (((cas - pi * n) ** 2)[m] / n[m]).sum(), pi
+
+
+def mi_upper(cas, ctl, k=None):
+    s, pi = chi_sum(cas, ctl)
+    if k is None:
+        k = curvature(pi)
+    N = float(np.sum(cas) + np.sum(ctl))
+    return k * s / (N * math.log(2.0))
 
 
 def bound(cas, ctl, k=None):
@@ -86,3 +103,20 @@ def test_bound_is_tight_for_a_separating_table():
     assert bound(cas, ctl) <= k2
     pi = cas.sum() / (cas.sum() + ctl.sum())
     assert bound(cas, ctl, k=curvature(pi) / 2) > k2
+
+
+def test_mi_upper_bound():
+    rng = np.random.default_rng(8)
+    for _ in range(2000):
+        cells = 27 if rng.integers(0, 4) else 9
+        cas, ctl = random_table(rng, cells)
+        mi = O.score_mi(cas, ctl)
+        assert mi <= mi_upper(cas, ctl) + 1e-12, (cas, ctl)
+
+
+def test_mi_bound_is_tight_for_a_separating_table():
+    cas = np.array([500] * 13 + [0] * 14, dtype=np.uint32)
+    ctl = np.array([0] * 13 + [464] * 13 + [468], dtype=np.uint32)
+    mi = O.score_mi(cas, ctl)                # 1 bit: the table separates the classes
+    assert abs(mi - 1.0) < 1e-12 and mi <= mi_upper(cas, ctl) + 1e-12
+    assert mi_upper(cas, ctl, k=curvature(0.5) / 2) < mi
