@@ -245,6 +245,8 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
         float bad[4];
         unsigned long long packed[4];
         unsigned long long hsum = 0;
+        float skip_above = INFINITY;     // see skip_threshold
+        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int q4 = 0; q4 < 4; q4++) {
             const int T = q4 * 512 + tid;
@@ -270,6 +272,8 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
             uint32_t cas[27], ctl[27];
             triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
             if constexpr (scores_sets(MODE)) {
+                if (skip_above < INFINITY && badness_lower_bound<27>(p.sc, cas, ctl) > skip_above)
+                    continue;                           // bad[q4] stays INFINITY
                 bad[q4] = badness_f32(p.sc, cas, ctl, s_lnf);
             } else if constexpr (MODE == MODE_DUMP) {
                 uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
@@ -517,6 +521,8 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
         float bad[8];
         unsigned long long packed[8];
         unsigned long long hsum = 0;
+        float skip_above = INFINITY;     // see skip_threshold
+        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int hf = 0; hf < 2; hf++) {
             // tiles with column blocks 2hf, 2hf+1 -> LDS [cls][wy][t4][reg][lane]
@@ -554,6 +560,8 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {
+                    if (skip_above < INFINITY && badness_lower_bound<27>(p.sc, cas, ctl) > skip_above)
+                        continue;                           // bad[m] stays INFINITY
                     bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                 } else if constexpr (MODE == MODE_DUMP) {
                     uint32_t *o = p.dump + rank3(X, Y, Z) * 54;

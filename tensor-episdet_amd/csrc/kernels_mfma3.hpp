@@ -166,6 +166,8 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma3(BatchTri bt)
         float bad[8];
         unsigned long long packed[8];
         unsigned long long hsum = 0;
+        float skip_above = INFINITY;     // see skip_threshold
+        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int hf = 0; hf < 2; hf++) {
             // column block hf of every row block -> LDS [cls][wy][rb][reg][lane]
@@ -204,6 +206,8 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma3(BatchTri bt)
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {
+                    if (skip_above < INFINITY && badness_lower_bound<27>(p.sc, cas, ctl) > skip_above)
+                        continue;                           // bad[m] stays INFINITY
                     bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                 } else if constexpr (MODE == MODE_DUMP) {
                     uint32_t *o = p.dump + rank3(X, Y, Z) * 54;

@@ -329,6 +329,8 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
         float bad[R * R];
         unsigned long long packed[R * R];
         unsigned long long hsum = 0;
+        float skip_above = INFINITY;     // see skip_threshold
+        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int A = a0 + ta + 16 * r;
@@ -351,6 +353,8 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
                     pair9(acc1[r][s][0], acc1[r][s][1], acc1[r][s][2], acc1[r][s][3], sa.z, sa.w, sb.z, sb.w,
                           (uint32_t)p.N1, ctl);
                     if constexpr (scores_sets(MODE)) {
+                        if (skip_above < INFINITY && badness_lower_bound<9>(p.sc, cas, ctl) > skip_above)
+                            continue;                           // bad[m] stays INFINITY
                         bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {
                         uint32_t *o = p.dump + rank2(A, B) * 18;
@@ -573,6 +577,8 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
             float bad[A * B];
             unsigned long long packed[A * B];
             unsigned long long hsum = 0;
+            float skip_above = INFINITY;     // see skip_threshold
+            if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
             for (int a = 0; a < A; a++) {
                 const int Y = y0 + ty + 16 * a;
@@ -588,6 +594,8 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
                     uint32_t cas[27], ctl[27];
                     triplet_tables(p, X, Y, Z, sx, sy, acc0[a][b], acc1[a][b], cas, ctl);
                     if constexpr (scores_sets(MODE)) {
+                        if (skip_above < INFINITY && badness_lower_bound<27>(p.sc, cas, ctl) > skip_above)
+                            continue;                           // bad[m] stays INFINITY
                         bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {
                         uint32_t *o = p.dump + rank3(X, Y, Z) * 54;
