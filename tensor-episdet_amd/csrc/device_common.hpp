@@ -274,8 +274,9 @@ __device__ __forceinline__ float key2f(unsigned k)
     return __uint_as_float(u);
 }
 
-// The badness above which a set's FP32 score may be skipped: the best
-// FP32 badness published so far (the global best only decreases, so a
+// The badness above which a set's FP32 score may be skipped: the global
+// bound published so far (single best: the best FP32 badness; top-k: an
+// upper bound of the k-th best, see publish_topk; it only decreases, so a
 // stale read only skips less) plus the slack, or +inf while no bound can
 // exceed it (bounds never exceed lb_max).
 __device__ __forceinline__ float skip_threshold(const ScoreCtx &sc, const unsigned *best_key)

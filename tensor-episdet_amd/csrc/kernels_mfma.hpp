@@ -246,7 +246,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
         unsigned long long packed[4];
         unsigned long long hsum = 0;
         float skip_above = INFINITY;     // see skip_threshold
-        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
+        if constexpr (scores_sets(MODE)) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int q4 = 0; q4 < 4; q4++) {
             const int T = q4 * 512 + tid;
@@ -522,7 +522,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
         unsigned long long packed[8];
         unsigned long long hsum = 0;
         float skip_above = INFINITY;     // see skip_threshold
-        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
+        if constexpr (scores_sets(MODE)) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int hf = 0; hf < 2; hf++) {
             // tiles with column blocks 2hf, 2hf+1 -> LDS [cls][wy][t4][reg][lane]

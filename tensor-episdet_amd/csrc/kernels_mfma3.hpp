@@ -167,7 +167,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma3(BatchTri bt)
         unsigned long long packed[8];
         unsigned long long hsum = 0;
         float skip_above = INFINITY;     // see skip_threshold
-        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
+        if constexpr (scores_sets(MODE)) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int hf = 0; hf < 2; hf++) {
             // column block hf of every row block -> LDS [cls][wy][rb][reg][lane]

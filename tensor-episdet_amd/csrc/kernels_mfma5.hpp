@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma5(BatchTri bt)
         // far by more than the slack cannot become candidates; their FP32
         // score is skipped (skip_threshold).
         float skip_above = INFINITY;
-        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
+        if constexpr (scores_sets(MODE)) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int hf = 0; hf < 2; hf++) {
             // column block hf -> LDS [class][wave][row block][reg][lane]

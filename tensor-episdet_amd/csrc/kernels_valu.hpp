@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
         unsigned long long packed[R * R];
         unsigned long long hsum = 0;
         float skip_above = INFINITY;     // see skip_threshold
-        if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
+        if constexpr (scores_sets(MODE)) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int A = a0 + ta + 16 * r;
@@ -578,7 +578,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
             unsigned long long packed[A * B];
             unsigned long long hsum = 0;
             float skip_above = INFINITY;     // see skip_threshold
-            if constexpr (MODE == MODE_SEARCH) skip_above = skip_threshold(p.sc, p.co.best_key);
+            if constexpr (scores_sets(MODE)) skip_above = skip_threshold(p.sc, p.co.best_key);
 #pragma unroll
             for (int a = 0; a < A; a++) {
                 const int Y = y0 + ty + 16 * a;
