@@ -139,7 +139,9 @@ typedef struct {
     double margin;              /* FP32 candidate margin used (2 x error bound)  */
     int fallback;               /* 1 if the exact fallback ran                   */
     int tile;                   /* kernel tile id used                           */
-    int reserved[6];
+    int seeded;                 /* 3-way fused: 1 if the search started from a
+                                   bound found through the best pairs          */
+    int reserved[5];
 } episdet_result;
 
 int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result *res);

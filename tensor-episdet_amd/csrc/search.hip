@@ -688,6 +688,7 @@ struct FusedPlan {
     unsigned long long *d_gstart = nullptr;
     int2 *d_groups = nullptr;
     unsigned long long nunits = 0;
+    unsigned seed_key = 0;       // starting global bound (best32), 0: none (see seed_from_pairs)
     FusedPlan() = default;
     FusedPlan(const FusedPlan &) = delete;
     FusedPlan &operator=(const FusedPlan &) = delete;
@@ -761,6 +762,8 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
     ta.nunits = pa.nunits = u1;
     CHK(init_ctl(ds));
     if (u0) HIPCHK(hipMemcpyAsync(&ds->ctl->queue, &u0, sizeof u0, hipMemcpyHostToDevice, ds->stream));
+    if (pl.seed_key)
+        HIPCHK(hipMemcpyAsync(&ds->ctl->best32, &pl.seed_key, sizeof pl.seed_key, hipMemcpyHostToDevice, ds->stream));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -792,6 +795,72 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
 }
 
 int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune);
+
+// Order-preserving key of a float, as f2key in the kernels.
+unsigned f2key_host(float f)
+{
+    unsigned u;
+    std::memcpy(&u, &f, sizeof u);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// A starting global bound for a single-best 3-way search: the best exact
+// badness among the triplets that contain one of the SEED_PAIRS best pairs
+// (a 2-way top-k search of the same data), plus half the candidate margin.
+// Every FP32 badness is within that half margin of the exact one, so the set
+// that attains the seed scores at or below it in FP32: the seed is a bound
+// some set reaches, as a published block minimum would be, and the winner
+// and every set tied with it still become candidates.  With it the score
+// bounds skip FP32 scores from the first unit on when strongly associated
+// triplets have associated pairs; without such pairs it only costs the
+// pre-pass.  Returns false, leaving the search unseeded, on any failure.
+constexpr int SEED_PAIRS = 4;
+bool seed_from_pairs(episdet_dataset *ds, int score, float margin, unsigned *key)
+{
+    if (ds->n < 4) return false;
+    episdet_options o;
+    std::memset(&o, 0, sizeof o);
+    o.order = 2;
+    o.score = score;
+    o.path = EPISDET_PATH_AUTO;
+    unsigned long long pk[SEED_PAIRS];
+    double sc[SEED_PAIRS];
+    int cnt = 0;
+    episdet_result r;
+    if (episdet_run_topk(ds, &o, SEED_PAIRS, pk, sc, &cnt, &r) != EPISDET_OK || cnt < 1) return false;
+    std::vector<unsigned long long> sets;
+    sets.reserve((size_t)cnt * ds->n);
+    for (int i = 0; i < cnt && i < SEED_PAIRS; i++) {
+        const int a = (int)(pk[i] & 0xFFFFFFFFu), b = (int)(pk[i] >> 32);
+        for (int c = 0; c < ds->n; c++) {
+            if (c == a || c == b) continue;
+            int t[3] = {a, b, c};
+            std::sort(t, t + 3);
+            sets.push_back((unsigned long long)t[0] | ((unsigned long long)t[1] << 21) |
+                           ((unsigned long long)t[2] << 42));
+        }
+    }
+    if (sets.empty() || ensure_cand(ds, (unsigned)sets.size()) != EPISDET_OK) return false;
+    if (hipMemcpy(ds->cand, sets.data(), sizeof(unsigned long long) * sets.size(), hipMemcpyHostToDevice) !=
+        hipSuccess)
+        return false;
+    double ms = 0;
+    if (rescore(ds, 3, score, (int)sets.size(), &ms) != EPISDET_OK) return false;
+    std::vector<double> s64(sets.size());
+    if (hipMemcpy(s64.data(), ds->cand_s64, sizeof(double) * s64.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return false;
+    double best = INFINITY;
+    for (double v : s64) {
+        if (std::isnan(v)) return false;
+        best = std::min(best, score == EPISDET_SCORE_K2 ? v : -v);     // badness: lower is better
+    }
+    const double bound = best + 0.5 * (double)margin;
+    float f = (float)bound;
+    if ((double)f < bound) f = std::nextafter(f, INFINITY);
+    if (!std::isfinite(f)) return false;
+    *key = f2key_host(f);
+    return true;
+}
 
 // ------------------------------------------------------------ autotuner --
 //
@@ -1016,6 +1085,11 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     FusedPlan pl;
     CHK(plan_fused(ds, order, score, path, tile, pl));
     const ScoreCtx &sc = pl.sc;
+    if (order == 3 && !topk) {
+        unsigned key = 0;
+        if (seed_from_pairs(ds, score, sc.margin, &key)) pl.seed_key = key;
+    }
+    res->seeded = pl.seed_key != 0;
 
     // One search launch; with top-k, a candidate overflow grows the buffer
     // and launches again (at most twice).

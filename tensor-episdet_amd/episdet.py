@@ -66,7 +66,8 @@ class Result(C.Structure):
                 ("sets", C.c_longlong), ("rounds", C.c_longlong), ("rounds_run", C.c_longlong),
                 ("candidates", C.c_longlong), ("kernel_ms", C.c_double), ("rescore_ms", C.c_double), ("prep_ms", C.c_double),
                 ("wall_s", C.c_double), ("fp32_max_abs_err", C.c_double), ("fp32_max_rel_err", C.c_double),
-                ("margin", C.c_double), ("fallback", C.c_int), ("tile", C.c_int), ("reserved", C.c_int * 6)]
+                ("margin", C.c_double), ("fallback", C.c_int), ("tile", C.c_int), ("seeded", C.c_int),
+                ("reserved", C.c_int * 5)]
 
     def as_dict(self):
         d = {f: getattr(self, f) for f, _ in self._fields_ if f not in ("reserved", "pad0")}
