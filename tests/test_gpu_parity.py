@@ -118,6 +118,25 @@ def test_mfma_versions_agree(tile):
         check_best(ds, syn.cases, syn.controls, 2500, 2100, 3, episdet.K2, path=episdet.PATH_FUSED_MFMA, tile=tile)
 
 
+def test_seeded_search_keeps_the_exact_winner():
+    """3-way single-best searches start from a bound found through the 4 best
+    pairs (seeded = 1); the winner and its FP64 score stay the oracle's, with a
+    planted triplet and with no planted set at all.  Fewer than 4 SNPs: no
+    seed."""
+    for planted, seed in ((1, 71), (0, 72)):
+        syn = episdet.generate(160, 2500, 2500, seed=seed, planted_sets=planted)
+        with episdet.Dataset.from_synthetic(syn) as ds:
+            for path in (episdet.PATH_FUSED_MFMA, episdet.PATH_FUSED_VALU):
+                r = check_best(ds, syn.cases, syn.controls, 2500, 2500, 3, episdet.K2, path=path)
+                assert r["seeded"] == 1
+            if planted:
+                assert tuple(r["idx"]) == syn.planted[0]
+    syn = episdet.generate(3, 400, 300, seed=73)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        r = check_best(ds, syn.cases, syn.controls, 400, 300, 3, episdet.K2, path=episdet.PATH_FUSED_MFMA)
+        assert r["seeded"] == 0
+
+
 def test_auto_runs_sequenced_classes_for_uneven_classes():
     """AUTO runs version 5 (tile 5, classes in sequence) when one class has
     under 65% of the other's words, version 4 (tile 4) otherwise; both give
