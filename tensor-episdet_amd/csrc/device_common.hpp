@@ -20,10 +20,12 @@ struct ScoreCtx {
     float hY;           // MI: H(Y) in bits
     float margin;       // candidate margin, 2 x the FP32 error bound
     // Lower bound of the badness (badness_lower_bound): lb_max - lb_scale * S
-    // with S the table's chi-square sum and pi = Nc/N; lb_slack is what a
-    // bound must clear above the best score before a set's FP32 score may
-    // be skipped (2 margins plus the bound's own FP32 error)
-    float lb_pi, lb_max, lb_scale, lb_slack;
+    // + lb_lnw * sum_c log2 max(n_c, 1) + lb_off, with S the table's
+    // chi-square sum and pi = Nc/N (lb_lnw = lb_off = 0 for MI); lb_top is
+    // the largest value the bound can take; lb_slack is what a bound must
+    // clear above the best score before a set's FP32 score may be skipped
+    // (2 margins plus the bound's own FP32 error)
+    float lb_pi, lb_max, lb_scale, lb_slack, lb_lnw, lb_off, lb_top;
 };
 
 // Single-SNP counts of both classes: {cases c0, cases c1, controls c0,
@@ -184,9 +186,22 @@ __device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl
 //    cells the linear terms cancel:  K2 >= N H(pi) - K S  (nats).
 //  * MI: I(G;Y) = sum_c (n_c/N) KL(p_c || pi) <= K S / N nats, so
 //    -MI >= -K S / (N ln 2) in bits.
-// Both are lb_max - lb_scale * S.  Cells with n = 0 add nothing.  NC is
-// the cell count (27 or 9, equal to sc.ncells), fixed at compile time so
-// the short loop unrolls.
+// Both are lb_max - lb_scale * S.  Cells with n = 0 add nothing.
+//  * K2 also carries each cell's Stirling term: for n >= 1,
+//      ln((n+1)!/(a! b!)) >= n H(a/n) + ln(n)/2 - c0,  c0 = ln(pi/2)/2,
+//    (with Robbins' ln m! = m ln m - m + ln(2 pi m)/2 + r_m, 1/(12m+1) < r_m
+//    < 1/(12m), the left side minus n H - ln(n)/2 is, for a, b >= 1,
+//    ln(n+1) - ln(2 pi a b)/2 + r_n - r_a - r_b; it exceeds -c0 because
+//    ln((n+1)/(2 sqrt(ab))) beats r_a + r_b < 1/(6m), m = min(a, b): it is
+//    at least 1/(n+1) (4ab <= n^2) when 6m >= n+1, else above ln(1.5)/2
+//    (4ab <= 4mn).  a = 0 or b = 0 gives ln(n+1) >= ln(n)/2.)  The infimum is
+//    approached at a = b as n grows.  Summed over the cells it adds
+//    sum_c ln(max(n_c, 1))/2 - NC c0, the same for every set of a given
+//    cell occupancy, so the bound can skip sets on data whose best set is
+//    weak.  The logarithms are taken of products of 5 cells (each below
+//    2^25, the product below 2^125) with v_log_f32.
+// NC is the cell count (27 or 9, equal to sc.ncells), fixed at compile
+// time so the short loops unroll.
 template <int NC>
 __device__ __forceinline__ float badness_lower_bound(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl)
 {
@@ -197,7 +212,22 @@ __device__ __forceinline__ float badness_lower_bound(const ScoreCtx &sc, const u
         const float n = (float)nn, d = fmaf(-sc.lb_pi, n, (float)cas[i]);
         s += nn ? d * d * __builtin_amdgcn_rcpf(n) : 0.f;
     }
-    return fmaf(-sc.lb_scale, s, sc.lb_max);
+    float lb = fmaf(-sc.lb_scale, s, sc.lb_max);
+    if (sc.lb_lnw != 0.f) {
+        float l = 0.f;
+#pragma unroll
+        for (int g = 0; g < NC; g += 5) {
+            float p = 1.f;
+#pragma unroll
+            for (int i = g; i < (g + 5 < NC ? g + 5 : NC); i++) {
+                const uint32_t nn = cas[i] + ctl[i];
+                p *= (float)(nn ? nn : 1u);
+            }
+            l += __builtin_amdgcn_logf(p);
+        }
+        lb = fmaf(sc.lb_lnw, l, lb + sc.lb_off);
+    }
+    return lb;
 }
 
 // "Badness": the FP32 score mapped so that lower is always better.
@@ -278,11 +308,11 @@ __device__ __forceinline__ float key2f(unsigned k)
 // bound published so far (single best: the best FP32 badness; top-k: an
 // upper bound of the k-th best, see publish_topk; it only decreases, so a
 // stale read only skips less) plus the slack, or +inf while no bound can
-// exceed it (bounds never exceed lb_max).
+// exceed it (bounds never exceed lb_top).
 __device__ __forceinline__ float skip_threshold(const ScoreCtx &sc, const unsigned *best_key)
 {
     const float t = key2f(__atomic_load_n(best_key, __ATOMIC_RELAXED)) + sc.lb_slack;
-    return t < sc.lb_max ? t : INFINITY;
+    return t < sc.lb_top ? t : INFINITY;
 }
 
 __device__ __forceinline__ unsigned long long d2key(double d)

@@ -235,25 +235,38 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
         eps = 2.0 * t.term_err + u * (3.0 * ncells * 0.531 * 2.0 + 8.0 * hmax) + 4.0 * u;
     }
     sc.margin = (float)(2.0 * eps);
-    // Badness lower bound (badness_lower_bound): lb_max - lb_scale * S.
+    // Badness lower bound (badness_lower_bound): lb_max - lb_scale * S, and
+    // for K2 + lb_lnw * sum_c log2 max(n_c, 1) + lb_off.
     // Its FP32 error: each (cas - pi n)^2 / n errs by about 2 |d| ulp(N) / n
     // <= 2 ulp(N) since |d| <= n (taken 4 times over), lb_scale times that
     // over the cells; K2's N H(pi) rounds by under 1e-7 N, and a further
-    // absolute term covers the sum's roundings.
+    // absolute term covers the sums' roundings and the log2 of the cell
+    // products (each product within 4 roundings, each log2 below 125 within
+    // a few ulps: under 1e-5 relative of the log sum, 25 ncells at most).
     const double K = std::max(-std::log1p(-qc) / (qc * qc), -std::log(qc) / (qt * qt)) * (1.0 + 1e-6);
     sc.lb_pi = (float)qc;
-    double scale, lbmax, extra;
+    double scale, lbmax, extra, lnw = 0.0, off = 0.0, top;
     if (score == EPISDET_SCORE_K2) {
         scale = K;
         lbmax = -(double)N * (qc * std::log(qc) + qt * std::log(qt));
-        extra = 1e-7 * N + 1.0;
+        lnw = 0.5 * std::log(2.0);
+        off = -ncells * 0.5 * std::log(M_PI / 2.0) * (1.0 + 1e-6);
+        // sum_c ln max(n_c, 1) over cells adding up to N is at most
+        // ncells ln(N / ncells) when N >= e ncells, else N / e
+        const double lnsum = N >= std::exp(1.0) * ncells ? ncells * std::log((double)N / ncells) : N / std::exp(1.0);
+        top = lbmax + 0.5 * lnsum + 1.0;
+        extra = 1e-7 * N + 1.0 + lnw * 25.0 * ncells * 1e-5;
     } else {
         scale = K / ((double)N * std::log(2.0));
         lbmax = 0.0;
+        top = 0.0;
         extra = 1e-6;
     }
     sc.lb_scale = (float)scale;
     sc.lb_max = (float)lbmax;
+    sc.lb_lnw = (float)lnw;
+    sc.lb_off = (float)off;
+    sc.lb_top = (float)top;
     sc.lb_slack = (float)(2.0 * sc.margin + scale * ncells * 8.0 * ulp32((double)N) + extra);
     return sc;
 }

@@ -6,6 +6,11 @@ S = sum_c (cas_c - pi n_c)^2 / n_c and K = max(-ln(1 - pi) / pi^2,
     K2 >= N H(pi) - K S          (nats; from C(n, k) >= e^{n H(k/n)} / (n + 1))
     MI <= K S / (N ln 2)         (bits; MI = sum_c (n_c / N) KL(p_c || pi))
 
+K2 further adds each cell's Stirling term, ln((n+1)!/(a! b!)) >= n H(a/n) +
+ln(n)/2 - c0 with c0 = ln(pi/2)/2 for n >= 1, so the kernels use
+
+    K2 >= N H(pi) - K S + sum_c ln(max(n_c, 1))/2 - ncells c0.
+
 Checked on the CPU against the oracle's exact FP64 K2 and MI over random,
 skewed and extreme tables; the curvature constant is checked to be the
 supremum it claims, and a halved constant is shown to break the bounds.
@@ -51,6 +56,26 @@ def bound(cas, ctl, k=None):
     s = (((cas - pi * n) ** 2)[m] / n[m]).sum()
     nh = -N * (pi * math.log(pi) + (1 - pi) * math.log(1 - pi))
     return nh - k * s
+
+
+C0 = 0.5 * math.log(math.pi / 2)
+
+
+def stirling_bound(cas, ctl, k=None, c0=C0, w=0.5):
+    n = np.asarray(cas, dtype=np.float64) + np.asarray(ctl, dtype=np.float64)
+    return bound(cas, ctl, k) + w * np.log(np.maximum(n, 1.0)).sum() - len(n) * c0
+
+
+def cell_gap(n):
+    """ln((n+1)!/(a! b!)) - n H(a/n) - ln(n)/2 for a = 0..n (b = n - a), exactly
+    as the oracle's K2 of a one-cell table."""
+    from scipy.special import gammaln
+    a = np.arange(0, n + 1, dtype=np.float64)
+    b = n - a
+    g = gammaln(n + 2) - gammaln(a + 1) - gammaln(b + 1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        nh = np.where(a > 0, -a * np.log(a / n), 0.0) + np.where(b > 0, -b * np.log(b / n), 0.0)
+    return g - nh - 0.5 * math.log(n)
 
 
 def random_table(rng, cells=27):
@@ -120,3 +145,62 @@ def test_mi_bound_is_tight_for_a_separating_table():
     mi = O.score_mi(cas, ctl)                # 1 bit: the table separates the classes
     assert abs(mi - 1.0) < 1e-12 and mi <= mi_upper(cas, ctl) + 1e-12
     assert mi_upper(cas, ctl, k=curvature(0.5) / 2) < mi
+
+
+def test_cell_stirling_term_exhaustive():
+    # every (a, b) with n = a + b <= 3000: the gap stays above -c0, and the
+    # one-cell gap matches the oracle's K2 of that cell on a sample
+    worst = min(cell_gap(n).min() for n in range(1, 3001))
+    assert worst > -C0
+    for a, b in ((0, 1), (1, 1), (3, 7), (250, 250), (1, 2999)):
+        cas, ctl = np.array([a], dtype=np.uint32), np.array([b], dtype=np.uint32)
+        n = a + b
+        h = sum(-x * math.log(x / n) for x in (a, b) if x)
+        assert abs(O.score_k2(cas, ctl) - h - 0.5 * math.log(n) - cell_gap(n)[a]) < 1e-9 * n
+
+
+def test_cell_stirling_constant_is_the_infimum():
+    # at a = b the gap is -c0 + about 0.75 / n: a constant 1e-3 smaller, or
+    # a log weight 1% larger, fails for large balanced cells
+    g = cell_gap(20000)[10000]
+    assert -C0 < g < -C0 + 1e-4
+    assert g < -(C0 - 1e-3)
+    assert cell_gap(200000)[100000] - 0.005 * math.log(200000) < -C0
+
+
+def test_stirling_bound_below_exact_k2():
+    rng = np.random.default_rng(11)
+    for _ in range(2000):
+        cells = 27 if rng.integers(0, 4) else 9
+        cas, ctl = random_table(rng, cells)
+        k2 = O.score_k2(cas, ctl)
+        assert stirling_bound(cas, ctl) <= k2 + 1e-9 * max(1.0, abs(k2)), (cas, ctl)
+
+
+def test_stirling_bound_is_tight_for_balanced_null_tables():
+    # 27 large cells with cases = controls: S = 0 and K2 exceeds the bound by
+    # only sum_c 0.75 / n_c; a constant 1e-3 smaller breaks it
+    cas = np.full(27, 5000, dtype=np.uint32)
+    ctl = np.full(27, 5000, dtype=np.uint32)
+    k2 = O.score_k2(cas, ctl)
+    lb = stirling_bound(cas, ctl)
+    assert lb <= k2 < lb + 27 * 1e-4
+    assert stirling_bound(cas, ctl, c0=C0 - 1e-3) > k2
+
+
+def test_stirling_term_lifts_the_bound_above_weak_winners():
+    # null data: the plain bound of a typical set lies below the K2 of the
+    # best of many random sets, so nothing could be skipped; the Stirling
+    # term puts typical sets above it
+    rng = np.random.default_rng(12)
+    Nc = Nt = 8192
+    tables = []
+    for _ in range(400):
+        p = rng.dirichlet(np.full(27, 2.0))
+        tables.append((rng.multinomial(Nc, p), rng.multinomial(Nt, p)))
+    k2 = np.array([O.score_k2(c, t) for c, t in tables])
+    best = k2.min()
+    plain = np.array([bound(c, t) for c, t in tables])
+    lifted = np.array([stirling_bound(c, t) for c, t in tables])
+    assert (plain < best).all()
+    assert (lifted > best).mean() > 0.5

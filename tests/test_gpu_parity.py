@@ -137,6 +137,32 @@ def test_seeded_search_keeps_the_exact_winner():
         assert r["seeded"] == 0
 
 
+NULL_KERNELS = [(3, episdet.PATH_FUSED_MFMA, t) for t in range(6)] + [(3, episdet.PATH_FUSED_VALU, 0)] + \
+    [(2, episdet.PATH_FUSED_MFMA, t) for t in range(2)] + [(2, episdet.PATH_FUSED_VALU, 0)]
+
+
+_NULL_RANKED = {}
+
+
+@pytest.mark.parametrize("order,path,tile", NULL_KERNELS)
+def test_null_data_k2_bound_keeps_the_winner(order, path, tile):
+    """No planted set: K2 winners are weak, and the score bound skips most
+    sets only through its per-cell Stirling term.  Single best and top-k
+    (k-th-best bound) stay the oracle's, FP64 scores bit-identical."""
+    n, Nc, Nt = (150, 2100, 1900) if order == 3 else (600, 1700, 2300)
+    if order not in _NULL_RANKED:              # the oracle's full ranking, once per order
+        syn = episdet.generate(n, Nc, Nt, seed=81 + order, planted_sets=0)
+        _NULL_RANKED[order] = (syn, O.ranked(syn.cases, syn.controls, Nc, Nt, order, episdet.K2))
+    syn, (ref_p, ref_s) = _NULL_RANKED[order]
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        check_best(ds, syn.cases, syn.controls, Nc, Nt, order, episdet.K2, path=path, tile=tile)
+        for k in (10, 200):
+            say(f"  null topk order={order} path={path} tile={tile} k={k}")
+            p, sc, _ = ds.topk(order, k, score=episdet.K2, path=path, tile=tile)
+            assert p == [int(x) for x in ref_p[:k]]
+            assert sc == [float(x) for x in ref_s[:k]]
+
+
 def test_auto_runs_sequenced_classes_for_uneven_classes():
     """AUTO runs version 5 (tile 5, classes in sequence) when one class has
     under 65% of the other's words, version 4 (tile 4) otherwise; both give
