@@ -83,7 +83,8 @@ int episdet_search(int order, int score, const unsigned *cases, const unsigned *
 typedef struct episdet_dataset episdet_dataset;
 
 /* Uploads both classes to device 0 and keeps them resident.
- * casesWords / controlsWords: W of the given host layout (>= ceil(N/32)). */
+ * casesWords / controlsWords: W of the given host layout (>= ceil(N/32)).
+ * numCases + numControls must stay below 2^30 (EPISDET_ERR_INVALID). */
 int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int numSNPs, int numCases,
                            int numControls, int casesWords, int controlsWords, episdet_dataset **out);
 void episdet_dataset_destroy(episdet_dataset *ds);

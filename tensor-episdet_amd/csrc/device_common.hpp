@@ -199,7 +199,8 @@ __device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl
 //    sum_c ln(max(n_c, 1))/2 - NC c0, the same for every set of a given
 //    cell occupancy, so the bound can skip sets on data whose best set is
 //    weak.  The logarithms are taken of products of 5 cells (each below
-//    2^25, the product below 2^125) with v_log_f32.
+//    2^25, the product below 2^125; the host sets lb_lnw = 0 for larger N)
+//    with v_log_f32.
 // NC is the cell count (27 or 9, equal to sc.ncells), fixed at compile
 // time so the short loops unroll.
 template <int NC>

@@ -249,13 +249,20 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
     if (score == EPISDET_SCORE_K2) {
         scale = K;
         lbmax = -(double)N * (qc * std::log(qc) + qt * std::log(qt));
-        lnw = 0.5 * std::log(2.0);
-        off = -ncells * 0.5 * std::log(M_PI / 2.0) * (1.0 + 1e-6);
-        // sum_c ln max(n_c, 1) over cells adding up to N is at most
-        // ncells ln(N / ncells) when N >= e ncells, else N / e
-        const double lnsum = N >= std::exp(1.0) * ncells ? ncells * std::log((double)N / ncells) : N / std::exp(1.0);
-        top = lbmax + 0.5 * lnsum + 1.0;
-        extra = 1e-7 * N + 1.0 + lnw * 25.0 * ncells * 1e-5;
+        top = lbmax;
+        extra = 1e-7 * N + 1.0;
+        // The Stirling term multiplies 5 cells in FP32: only while every
+        // cell, at most N, is below 2^25 (the VALU path takes larger N).
+        if (N < (1 << 25)) {
+            lnw = 0.5 * std::log(2.0);
+            off = -ncells * 0.5 * std::log(M_PI / 2.0) * (1.0 + 1e-6);
+            // sum_c ln max(n_c, 1) over cells adding up to N is at most
+            // ncells ln(N / ncells) when N >= e ncells, else N / e
+            const double lnsum =
+                N >= std::exp(1.0) * ncells ? ncells * std::log((double)N / ncells) : N / std::exp(1.0);
+            top = lbmax + 0.5 * lnsum + 1.0;
+            extra += lnw * 25.0 * ncells * 1e-5;
+        }
     } else {
         scale = K / ((double)N * std::log(2.0));
         lbmax = 0.0;
@@ -1376,6 +1383,8 @@ int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int 
     *out = nullptr;
     if (!cases || !controls || numSNPs < 2 || numSNPs > 0x1FFFFF || numCases < 1 || numControls < 1)
         return EPISDET_ERR_INVALID;
+    // N = numCases + numControls, and N + 2 table entries, stay in int
+    if ((long long)numCases + numControls >= (1LL << 30)) return EPISDET_ERR_INVALID;
     if (casesWords * 32LL < numCases || controlsWords * 32LL < numControls) return EPISDET_ERR_INVALID;
     int cu = 0;
     CHK(check_device(&cu));

@@ -80,6 +80,8 @@ def test_invalid_arguments_rejected_before_device_use():
     # numSNPs < 2 and words too small for the sample count are argument errors
     assert lib.episdet_dataset_create(z, z, 1, 10, 10, 32, 32, ctypes.byref(h)) == 1
     assert lib.episdet_dataset_create(z, z, 4, 2000, 10, 32, 32, ctypes.byref(h)) == 1
+    # more than 2^30 samples in all (checked before the plane sizes)
+    assert lib.episdet_dataset_create(z, z, 4, 1 << 29, 1 << 29, 1 << 24, 1 << 24, ctypes.byref(h)) == 1
     # the extended entry points check their arguments before touching a device
     res = episdet.Result()
     opt = episdet.Options(order=3, score=0)
