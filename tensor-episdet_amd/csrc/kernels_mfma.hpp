@@ -142,6 +142,7 @@ struct BatchTri {
     int njobs;
     unsigned long long total;
     unsigned long long *queue;
+    unsigned sh = 1;                        // class-3 operand shift (see m2_aop)
 };
 
 template <int MODE>
@@ -400,8 +401,13 @@ __device__ __forceinline__ void m2_issue_buf(i32x4_t rsrc, const unsigned (&roff
 // Word offset of logical 16-byte slot q of LDS row r.
 __device__ __forceinline__ int m2_at(int r, int q) { return r * M2_KW + ((q ^ (r & 7)) << 2); }
 
+// A operand X & Y of class C.  Class 3 sits in the sign bit of each FP4
+// nibble and moves to bit 2: (x & y & 0x88888888) >> 1, one v_bitop3_b32 and
+// one shift.  The shift count sh (always 1) comes from the kernel arguments:
+// with a constant count the compiler rewrites the expression as
+// ((x & y) >> 1) & 0x44444444, three instructions.
 template <int C>
-__device__ __forceinline__ v8i_t m2_aop(const uint4 &x, const uint4 &y)
+__device__ __forceinline__ v8i_t m2_aop(const uint4 &x, const uint4 &y, unsigned sh)
 {
     v8i_t o = {0, 0, 0, 0, 0, 0, 0, 0};
     if constexpr (C < 3) {
@@ -411,20 +417,20 @@ __device__ __forceinline__ v8i_t m2_aop(const uint4 &x, const uint4 &y)
         o[2] = (int)(x.z & y.z & m);
         o[3] = (int)(x.w & y.w & m);
     } else {
-        o[0] = (int)(((x.x & y.x) >> 1) & 0x44444444u);
-        o[1] = (int)(((x.y & y.y) >> 1) & 0x44444444u);
-        o[2] = (int)(((x.z & y.z) >> 1) & 0x44444444u);
-        o[3] = (int)(((x.w & y.w) >> 1) & 0x44444444u);
+        o[0] = (int)((x.x & y.x & 0x88888888u) >> sh);
+        o[1] = (int)((x.y & y.y & 0x88888888u) >> sh);
+        o[2] = (int)((x.z & y.z & 0x88888888u) >> sh);
+        o[3] = (int)((x.w & y.w & 0x88888888u) >> sh);
     }
     return o;
 }
 
 template <int C>
 __device__ __forceinline__ void m2_step(const uint4 &xw, const uint4 (&yw)[2], const uint4 (&zw)[4],
-                                        v16f_t (&acc)[2][4])
+                                        v16f_t (&acc)[2][4], unsigned sh)
 {
     constexpr int s = mf_scale<C>();
-    const v8i_t A0 = m2_aop<C>(xw, yw[0]), A1 = m2_aop<C>(xw, yw[1]);
+    const v8i_t A0 = m2_aop<C>(xw, yw[0], sh), A1 = m2_aop<C>(xw, yw[1], sh);
 #pragma unroll
     for (int cb = 0; cb < 4; cb++) {
         const v8i_t B = mf_operand<C>(zw[cb]);
@@ -507,10 +513,10 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
                     yw[1] = *(const uint4 *)&cur[m2_at(yr1, q)];
 #pragma unroll
                     for (int cb = 0; cb < 4; cb++) zw[cb] = *(const uint4 *)&cur[m2_at(zr[cb], q)];
-                    m2_step<0>(xw, yw, zw, acc);
-                    m2_step<1>(xw, yw, zw, acc);
-                    m2_step<2>(xw, yw, zw, acc);
-                    m2_step<3>(xw, yw, zw, acc);
+                    m2_step<0>(xw, yw, zw, acc, bt.sh);
+                    m2_step<1>(xw, yw, zw, acc, bt.sh);
+                    m2_step<2>(xw, yw, zw, acc, bt.sh);
+                    m2_step<3>(xw, yw, zw, acc, bt.sh);
                 }
             }
             __builtin_amdgcn_s_waitcnt(0);

@@ -68,13 +68,13 @@ __device__ __forceinline__ int m3_at(int r, int q) { return r * M3_KW + ((q ^ ((
 
 template <int C>
 __device__ __forceinline__ void m3_step(const uint4 &xw, const uint4 (&yw)[4], const uint4 (&zw)[2],
-                                        v16f_t (&acc)[4][2])
+                                        v16f_t (&acc)[4][2], unsigned sh)
 {
     constexpr int s = mf_scale<C>();
     const v8i_t B0 = mf_operand<C>(zw[0]), B1 = mf_operand<C>(zw[1]);
 #pragma unroll
     for (int rb = 0; rb < 4; rb++) {
-        const v8i_t A = m2_aop<C>(xw, yw[rb]);
+        const v8i_t A = m2_aop<C>(xw, yw[rb], sh);
         acc[rb][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B0, acc[rb][0], 4, 4, 0, s, 0, s);
         acc[rb][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B1, acc[rb][1], 4, 4, 0, s, 0, s);
     }
@@ -152,10 +152,10 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma3(BatchTri bt)
                     for (int rb = 0; rb < 4; rb++) yw[rb] = *(const uint4 *)&cur[m3_at(yr[rb], q)];
                     zw[0] = *(const uint4 *)&cur[m3_at(zr0, q)];
                     zw[1] = *(const uint4 *)&cur[m3_at(zr1, q)];
-                    m3_step<0>(xw, yw, zw, acc);
-                    m3_step<1>(xw, yw, zw, acc);
-                    m3_step<2>(xw, yw, zw, acc);
-                    m3_step<3>(xw, yw, zw, acc);
+                    m3_step<0>(xw, yw, zw, acc, bt.sh);
+                    m3_step<1>(xw, yw, zw, acc, bt.sh);
+                    m3_step<2>(xw, yw, zw, acc, bt.sh);
+                    m3_step<3>(xw, yw, zw, acc, bt.sh);
                 }
             }
             __builtin_amdgcn_s_waitcnt(0);

@@ -177,10 +177,10 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
                     for (int rb = 0; rb < 4; rb++) yw[rb] = *(const uint4 *)&cur[G::at(yr[rb], q)];
                     zw[0] = *(const uint4 *)&cur[G::at(zr0, q)];
                     zw[1] = *(const uint4 *)&cur[G::at(zr1, q)];
-                    m3_step<0>(xw, yw, zw, acc);
-                    m3_step<1>(xw, yw, zw, acc);
-                    m3_step<2>(xw, yw, zw, acc);
-                    m3_step<3>(xw, yw, zw, acc);
+                    m3_step<0>(xw, yw, zw, acc, bt.sh);
+                    m3_step<1>(xw, yw, zw, acc, bt.sh);
+                    m3_step<2>(xw, yw, zw, acc, bt.sh);
+                    m3_step<3>(xw, yw, zw, acc, bt.sh);
                 }
             }
         }

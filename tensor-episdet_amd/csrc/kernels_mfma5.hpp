@@ -69,13 +69,13 @@ __device__ __forceinline__ void m5_issue_buf(i32x4_t rsrc, const unsigned (&roff
 
 template <int C>
 __device__ __forceinline__ void m5_step(const uint4 &xw, const uint4 (&yw)[2], const uint4 (&zw)[2],
-                                        v16f_t (&acc)[2][2])
+                                        v16f_t (&acc)[2][2], unsigned sh)
 {
     constexpr int s = mf_scale<C>();
     const v8i_t B0 = mf_operand<C>(zw[0]), B1 = mf_operand<C>(zw[1]);
 #pragma unroll
     for (int rb = 0; rb < 2; rb++) {
-        const v8i_t A = m2_aop<C>(xw, yw[rb]);
+        const v8i_t A = m2_aop<C>(xw, yw[rb], sh);
         acc[rb][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B0, acc[rb][0], 4, 4, 0, s, 0, s);
         acc[rb][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A, B1, acc[rb][1], 4, 4, 0, s, 0, s);
     }
@@ -83,7 +83,7 @@ __device__ __forceinline__ void m5_step(const uint4 &xw, const uint4 (&yw)[2], c
 
 // Contracts one staged chunk into one class's accumulators.
 __device__ __forceinline__ void m5_chunk(const uint32_t *cur, int xr, const int (&yr)[2], int zr0, int zr1, int h,
-                                         v16f_t (&acc)[2][2])
+                                         v16f_t (&acc)[2][2], unsigned sh)
 {
 #pragma unroll
     for (int s = 0; s < M5_KW / 8; s++) {
@@ -94,10 +94,10 @@ __device__ __forceinline__ void m5_chunk(const uint32_t *cur, int xr, const int 
         yw[1] = *(const uint4 *)&cur[m2_at(yr[1], q)];
         zw[0] = *(const uint4 *)&cur[m2_at(zr0, q)];
         zw[1] = *(const uint4 *)&cur[m2_at(zr1, q)];
-        m5_step<0>(xw, yw, zw, acc);
-        m5_step<1>(xw, yw, zw, acc);
-        m5_step<2>(xw, yw, zw, acc);
-        m5_step<3>(xw, yw, zw, acc);
+        m5_step<0>(xw, yw, zw, acc, sh);
+        m5_step<1>(xw, yw, zw, acc, sh);
+        m5_step<2>(xw, yw, zw, acc, sh);
+        m5_step<3>(xw, yw, zw, acc, sh);
     }
 }
 
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma5(BatchTri bt)
                     if (tn < nch0) m5_issue_buf(rs0, roff0, tn * M5_KW, nb, w);
                     else m5_issue_buf(rs1, roff1, (tn - nch0) * M5_KW, nb, w);
                 }
-                m5_chunk(smem + (t % M5_NBUF) * M5_BUF_WORDS, xr, yr, zr0, zr1, h, acc[c]);
+                m5_chunk(smem + (t % M5_NBUF) * M5_BUF_WORDS, xr, yr, zr0, zr1, h, acc[c], bt.sh);
             }
         }
         __builtin_amdgcn_s_waitcnt(0);
