@@ -19,10 +19,10 @@ struct ScoreCtx {
     float invN;         // 1/N
     float hY;           // MI: H(Y) in bits
     float margin;       // candidate margin, 2 x the FP32 error bound
-    // Lower bound of the badness (badness_lower_bound): lb_max - lb_scale * S
-    // + lb_lnw * sum_c log2 max(n_c, 1) + lb_off, with S the table's
-    // chi-square sum and pi = Nc/N (lb_lnw = lb_off = 0 for MI); lb_top is
-    // the largest value the bound can take; lb_slack is what a bound must
+    // Lower bounds of the badness (bound_exceeds): lb_max - lb_scale * S, and
+    // for K2 that plus lb_lnw * sum_c log2 max(n_c, 1) + lb_off, with S the
+    // table's chi-square sum and pi = Nc/N (lb_lnw = 0: no second bound);
+    // lb_top is the largest value either can take; lb_slack is what a bound must
     // clear above the best score before a set's FP32 score may be skipped
     // (2 margins plus the bound's own FP32 error)
     float lb_pi, lb_max, lb_scale, lb_slack, lb_lnw, lb_off, lb_top;
@@ -201,10 +201,11 @@ __device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl
 //    weak.  The logarithms are taken of products of 5 cells (each below
 //    2^25, the product below 2^125; the host sets lb_lnw = 0 for larger N)
 //    with v_log_f32.
-// NC is the cell count (27 or 9, equal to sc.ncells), fixed at compile
-// time so the short loops unroll.
+// bound_exceeds tells whether the bound of a table exceeds thr.  NC is the
+// cell count (27 or 9, equal to sc.ncells), fixed at compile time so the
+// short loops unroll.
 template <int NC>
-__device__ __forceinline__ float badness_lower_bound(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl)
+__device__ __forceinline__ bool bound_exceeds(const ScoreCtx &sc, const uint32_t *cas, const uint32_t *ctl, float thr)
 {
     float s = 0.f;
 #pragma unroll
@@ -213,22 +214,22 @@ __device__ __forceinline__ float badness_lower_bound(const ScoreCtx &sc, const u
         const float n = (float)nn, d = fmaf(-sc.lb_pi, n, (float)cas[i]);
         s += nn ? d * d * __builtin_amdgcn_rcpf(n) : 0.f;
     }
-    float lb = fmaf(-sc.lb_scale, s, sc.lb_max);
-    if (sc.lb_lnw != 0.f) {
-        float l = 0.f;
+    const float lb = fmaf(-sc.lb_scale, s, sc.lb_max);
+    // Without and with the Stirling term both are lower bounds: the logs
+    // are only taken for a set the first does not already rule out.
+    if (lb > thr || sc.lb_lnw == 0.f) return lb > thr;
+    float l = 0.f;
 #pragma unroll
-        for (int g = 0; g < NC; g += 5) {
-            float p = 1.f;
+    for (int g = 0; g < NC; g += 5) {
+        float p = 1.f;
 #pragma unroll
-            for (int i = g; i < (g + 5 < NC ? g + 5 : NC); i++) {
-                const uint32_t nn = cas[i] + ctl[i];
-                p *= (float)(nn ? nn : 1u);
-            }
-            l += __builtin_amdgcn_logf(p);
+        for (int i = g; i < (g + 5 < NC ? g + 5 : NC); i++) {
+            const uint32_t nn = cas[i] + ctl[i];
+            p *= (float)(nn ? nn : 1u);
         }
-        lb = fmaf(sc.lb_lnw, l, lb + sc.lb_off);
+        l += __builtin_amdgcn_logf(p);
     }
-    return lb;
+    return fmaf(sc.lb_lnw, l, lb + sc.lb_off) > thr;
 }
 
 // "Badness": the FP32 score mapped so that lower is always better.

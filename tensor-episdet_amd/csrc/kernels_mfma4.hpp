@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);
                 if constexpr (scores_sets(MODE)) {
-                    if (skip_above < INFINITY && badness_lower_bound<27>(p.sc, cas, ctl) > skip_above)
+                    if (skip_above < INFINITY && bound_exceeds<27>(p.sc, cas, ctl, skip_above))
                         continue;                           // bad[m] stays INFINITY
                     bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                 } else if constexpr (MODE == MODE_DUMP) {

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
                     pair9(n4c[0], n4c[1], n4c[2], n4c[3], sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
                     pair9(n4t[0], n4t[1], n4t[2], n4t[3], sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);
                     if constexpr (scores_sets(MODE)) {
-                        if (skip_above < INFINITY && badness_lower_bound<9>(p.sc, cas, ctl) > skip_above)
+                        if (skip_above < INFINITY && bound_exceeds<9>(p.sc, cas, ctl, skip_above))
                             continue;                           // bad[m] stays INFINITY
                         bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {

@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void k_pairs_valu(BatchPair bp)
                     pair9(acc1[r][s][0], acc1[r][s][1], acc1[r][s][2], acc1[r][s][3], sa.z, sa.w, sb.z, sb.w,
                           (uint32_t)p.N1, ctl);
                     if constexpr (scores_sets(MODE)) {
-                        if (skip_above < INFINITY && badness_lower_bound<9>(p.sc, cas, ctl) > skip_above)
+                        if (skip_above < INFINITY && bound_exceeds<9>(p.sc, cas, ctl, skip_above))
                             continue;                           // bad[m] stays INFINITY
                         bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {
@@ -594,7 +594,7 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
                     uint32_t cas[27], ctl[27];
                     triplet_tables(p, X, Y, Z, sx, sy, acc0[a][b], acc1[a][b], cas, ctl);
                     if constexpr (scores_sets(MODE)) {
-                        if (skip_above < INFINITY && badness_lower_bound<27>(p.sc, cas, ctl) > skip_above)
+                        if (skip_above < INFINITY && bound_exceeds<27>(p.sc, cas, ctl, skip_above))
                             continue;                           // bad[m] stays INFINITY
                         bad[m] = badness_f32(p.sc, cas, ctl, s_lnf);
                     } else if constexpr (MODE == MODE_DUMP) {

@@ -235,7 +235,7 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
         eps = 2.0 * t.term_err + u * (3.0 * ncells * 0.531 * 2.0 + 8.0 * hmax) + 4.0 * u;
     }
     sc.margin = (float)(2.0 * eps);
-    // Badness lower bound (badness_lower_bound): lb_max - lb_scale * S, and
+    // Badness lower bound (bound_exceeds): lb_max - lb_scale * S, and
     // for K2 + lb_lnw * sum_c log2 max(n_c, 1) + lb_off.
     // Its FP32 error: each (cas - pi n)^2 / n errs by about 2 |d| ulp(N) / n
     // <= 2 ulp(N) since |d| <= n (taken 4 times over), lb_scale times that

@@ -1,5 +1,5 @@
 """The score bounds with which the MFMA kernels skip FP32 scores that cannot
-matter (badness_lower_bound in device_common.hpp).  With pi = Nc / N,
+matter (bound_exceeds in device_common.hpp).  With pi = Nc / N,
 S = sum_c (cas_c - pi n_c)^2 / n_c and K = max(-ln(1 - pi) / pi^2,
 -ln(pi) / (1 - pi)^2), the largest KL(p || pi) / (p - pi)^2 on [0, 1]:
 
