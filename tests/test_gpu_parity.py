@@ -163,6 +163,24 @@ def test_null_data_k2_bound_keeps_the_winner(order, path, tile):
             assert sc == [float(x) for x in ref_s[:k]]
 
 
+def test_more_than_2_24_samples_per_class():
+    """Both classes above 2^24 samples (33.6 M in all): the FP4 path's FP32
+    counts would not be exact, so AUTO runs the VALU path and a forced MFMA
+    path is refused; the K2 bound runs without its Stirling term (N >= 2^25).
+    Winners and FP64 scores equal the oracle's for 2-/3-way, K2 and MI."""
+    Nc, Nt = (1 << 24) + 37, (1 << 24) + 1001
+    syn = episdet.generate(5, Nc, Nt, seed=91, planted_sets=0)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for order in (2, 3):
+            for score in (episdet.K2, episdet.MI):
+                r = check_best(ds, syn.cases, syn.controls, Nc, Nt, order, score, path=episdet.PATH_AUTO)
+                assert r["path"] == episdet.PATH_FUSED_VALU
+            say(f"  forced MFMA path at Nc={Nc}: refused")
+            with pytest.raises(episdet.EpisdetError) as e:
+                ds.run(order, episdet.K2, path=episdet.PATH_FUSED_MFMA)
+            assert e.value.code == 1
+
+
 def test_auto_runs_sequenced_classes_for_uneven_classes():
     """AUTO runs version 5 (tile 5, classes in sequence) when one class has
     under 65% of the other's words, version 4 (tile 4) otherwise; both give
