@@ -158,7 +158,7 @@ def bench_batch(args, episdet):
         "metric": "batched 2-/3-way K2/MI searches, total sets x samples/s (64 uneven jobs, one graph launch)",
         "value": setsamp / sec, "unit": "set-samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "3-way: fp4 0/1 MFMA operands, exact fp32 counts; 2-way: u32 AND+popcount; FP32 scoring, "
+        "dtype": "fp4 e2m1 0/1 MFMA operands, fp32 exact-integer accumulation (2- and 3-way); FP32 scoring, "
                  "FP64 re-score",
         "data": "synthetic",
         "config": {"workload": "64 jobs: order/score 50/50, SNPs log-U[256,8192] (3-way log-U[256,2048]), samples "
