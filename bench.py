@@ -158,8 +158,9 @@ def bench_batch(args, episdet):
         "metric": "batched 2-/3-way K2/MI searches, total sets x samples/s (64 uneven jobs, one graph launch)",
         "value": setsamp / sec, "unit": "set-samples/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp4 e2m1 0/1 MFMA operands, fp32 exact-integer accumulation (2- and 3-way); FP32 scoring, "
-                 "FP64 re-score",
+        "dtype": "fp4",
+        "dtype_detail": "fp4 e2m1 0/1 MFMA operands, fp32 exact-integer accumulation (2- and 3-way); FP32 scoring, "
+                        "FP64 re-score",
         "data": "synthetic",
         "config": {"workload": "64 jobs: order/score 50/50, SNPs log-U[256,8192] (3-way log-U[256,2048]), samples "
                                "U[8000,131072] split 50/50, tight padding, MAF~U[0.05,0.5]",
@@ -337,8 +338,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp4 e2m1 0/1 operands, fp32 exact-integer accumulation" if ran == "mfma" else
-                  "u32 bit-plane AND+popcount") + "; exact int counts, FP32 scoring, FP64 re-score",
+        "dtype": "fp4" if ran == "mfma" else "u32",
+        "dtype_detail": ("fp4 e2m1 0/1 operands, fp32 exact-integer accumulation" if ran == "mfma" else
+                         "u32 bit-plane AND+popcount") + "; exact int counts, FP32 scoring, FP64 re-score",
         "data": "synthetic",
         "config": {"workload": f"{order}-way {SCORE_NAME[score]} exhaustive search, {n} SNPs x {N} samples "
                                f"({Nc}/{Nt}), HWE, MAF~{pre['maf_text']}, {pre['planted']} planted "

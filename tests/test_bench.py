@@ -32,6 +32,7 @@ def test_bench_json_line():
     assert d["n_gpus"] == 1 and d["steps"] == 3 and d["warmup"] == 1
     assert d["value"] > 0 and d["ms_per_step"] > 0 and d["higher_is_better"] is True
     assert d["unit"] == "set-samples/s" and d["data"] == "synthetic" and "workload" in d["config"]
+    assert d["dtype"] == "fp4"                    # the FP4 MFMA path runs this shape
     r = d["roofline"]
     assert r["bound"] in ("hbm", "mfma") and r["unit"] in ("GB/s", "TFLOP/s") and r["peak"] > 0 and r["achieved"] > 0
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-9 and "traffic" in r
