@@ -314,6 +314,23 @@ def test_full_sample_axis_planted_triplet():
             assert (t[i, 1] == O.table_bits(syn.controls, 131072, sets[i])).all()
 
 
+def test_full_sample_axis_checksums_agree():
+    """The benchmark's sample axis (131072 + 131072) on 1024 SNPs (178,433,024
+    triplets, 1/64 of the benchmark's): the order-independent checksum of
+    every (set, 54-cell table) from the default kernel equals that of two
+    other tilings (tiles 0 and 5) and of the integer AND+popcount VALU path.
+    (At 4096 SNPs each checksum pass would hash 1.1e10 tables, minutes.)"""
+    syn = episdet.generate(1024, 131072, 131072, seed=1, planted_sets=1, planted_order=3)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        say("1024 x 262144 checksum: MFMA tile 4")
+        c4 = ds.checksum(3, path=episdet.PATH_FUSED_MFMA, tile=4)
+        for tile in (0, 5):
+            say(f"1024 x 262144 checksum: MFMA tile {tile}")
+            assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA, tile=tile) == c4
+        say("1024 x 262144 checksum: VALU")
+        assert ds.checksum(3, path=episdet.PATH_FUSED_VALU) == c4
+
+
 def test_mi_pairs_uneven_classes_planted():
     """Config-3 style: 40/60 classes with padding that is not a 1024-bit
     multiple of the valid bits, MAF ~ Beta(0.5, 2), planted pair."""
