@@ -331,6 +331,24 @@ def test_full_sample_axis_checksums_agree():
         assert ds.checksum(3, path=episdet.PATH_FUSED_VALU) == c4
 
 
+# Checksum of all 11,444,858,880 (set, table) pairs of the benchmark workload
+# (generate(4096, 131072, 131072, seed=1, planted_sets=1)): the value MFMA
+# tiles 4, 0 and 5 and the VALU path all returned in one run (VALU: 68 s).
+BENCH_WORKLOAD_CHECKSUM = 0x4F1B9FCBA564E85F
+
+
+def test_benchmark_workload_checksum():
+    """The benchmark workload at full size, 4096 SNPs x 262144 samples: the
+    order-independent checksum of every triplet's 54-cell table from the
+    default kernel (tile 4) and from tile 0 (64 x 64 units) equals the value
+    all four kernels agreed on."""
+    syn = episdet.generate(4096, 131072, 131072, seed=1, planted_sets=1, planted_order=3)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        for tile in (4, 0):
+            say(f"benchmark workload checksum: MFMA tile {tile} (about 20 s)")
+            assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA, tile=tile) == BENCH_WORKLOAD_CHECKSUM
+
+
 def test_mi_pairs_uneven_classes_planted():
     """Config-3 style: 40/60 classes with padding that is not a 1024-bit
     multiple of the valid bits, MAF ~ Beta(0.5, 2), planted pair."""
