@@ -349,6 +349,21 @@ def test_benchmark_workload_checksum():
             assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA, tile=tile) == BENCH_WORKLOAD_CHECKSUM
 
 
+def test_pair_workload_checksums_agree():
+    """The 2-way benchmark workload at full size (16384 SNPs, 26214 cases +
+    39322 controls, MAF ~ Beta(0.5, 2)): the checksum of all 134,209,536 pair
+    tables is the same from both MFMA pair kernels and the VALU path."""
+    syn = episdet.generate(16384, 26214, 39322, seed=1, planted_sets=1, planted_order=2,
+                           maf=(0.01, 0.5), maf_dist=episdet.MAF_BETA)
+    with episdet.Dataset.from_synthetic(syn) as ds:
+        say("2-way workload checksum: MFMA tile 1")
+        c1 = ds.checksum(2, path=episdet.PATH_FUSED_MFMA, tile=1)
+        say("2-way workload checksum: MFMA tile 0")
+        assert ds.checksum(2, path=episdet.PATH_FUSED_MFMA, tile=0) == c1
+        say("2-way workload checksum: VALU")
+        assert ds.checksum(2, path=episdet.PATH_FUSED_VALU) == c1
+
+
 def test_mi_pairs_uneven_classes_planted():
     """Config-3 style: 40/60 classes with padding that is not a 1024-bit
     multiple of the valid bits, MAF ~ Beta(0.5, 2), planted pair."""
