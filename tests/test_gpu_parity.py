@@ -341,12 +341,20 @@ def test_benchmark_workload_checksum():
     """The benchmark workload at full size, 4096 SNPs x 262144 samples: the
     order-independent checksum of every triplet's 54-cell table from the
     default kernel (tile 4) and from tile 0 (64 x 64 units) equals the value
-    all four kernels agreed on."""
+    all four kernels agreed on; the single-best and top-k searches agree."""
     syn = episdet.generate(4096, 131072, 131072, seed=1, planted_sets=1, planted_order=3)
     with episdet.Dataset.from_synthetic(syn) as ds:
         for tile in (4, 0):
             say(f"benchmark workload checksum: MFMA tile {tile} (about 20 s)")
             assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA, tile=tile) == BENCH_WORKLOAD_CHECKSUM
+        # single best (seeded, skipping against the best) and top-k (skipping
+        # against the k-th best) agree on the winner; it is the planted set
+        say("benchmark workload: single best and top-16")
+        r = ds.run(3, episdet.K2)
+        p, sc, _ = ds.topk(3, 16, score=episdet.K2)
+        assert r["seeded"] == 1 and r["packed"] == p[0] and r["best_score"] == sc[0]
+        assert tuple(r["idx"]) == syn.planted[0]
+        assert sc == sorted(sc) and len(set(p)) == 16
 
 
 def test_pair_workload_checksums_agree():
