@@ -40,7 +40,8 @@ enum episdet_status {
     EPISDET_ERR_HIP = 3,          /* a HIP runtime call failed                      */
     EPISDET_ERR_ALLOC = 4,        /* device or host allocation failed               */
     EPISDET_ERR_CANDIDATES = 5,   /* candidate buffer overflow that could not be
-                                     resolved by the exact fallback               */
+                                     resolved (not returned by this version: an
+                                     overflow reruns the search in windows)     */
     EPISDET_ERR_INTERNAL = 6,
     EPISDET_ERR_GENOTYPES = 7     /* a sample is set in both planes of a SNP: the
                                      input encodes no genotype for it             */
@@ -70,7 +71,11 @@ enum episdet_path {
  *   outputFromGpu        best score (K2 minimised, MI maximised), FP64 value
  *                        rounded to float.
  *   packedIndices        X | Y<<21 | Z<<42 (3-way) or A | B<<32 (2-way),
- *                        X < Y < Z; ties go to the lowest packed index. */
+ *                        X < Y < Z; ties go to the lowest packed index.
+ * Like the reference, it prints "Outer loop iteration i out of nb" to stdout,
+ * one line per block of 256 SNPs of numSNPsWithPadding (search-pairs.cu:139,
+ * search-triplets.cu:164); the lines come before the search, which is one
+ * launch. */
 int episdet_search(int order, int score, const unsigned *cases, const unsigned *controls, int numSNPs,
                    int numCases, int numControls, unsigned numSNPsWithPadding, int numCasesWithPadding,
                    int numControlsWithPadding, int *roundsCounter, double *searchTime, float *outputFromGpu,
@@ -103,7 +108,12 @@ typedef struct {
                              0-2; PLAIN ignores it                               */
     int plain_rounds;     /* PLAIN only: stop after this many rounds (0 = all);
                              used to time a contiguous slice of the round loop   */
-    int candidate_cap;    /* 0 = default (1<<20)                                 */
+    int candidate_cap;    /* FP32 candidates kept per launch for FP64 re-scoring;
+                             0 = default (1<<20).  Values below one work unit's
+                             sets (at most 8192) are raised to it.  When more
+                             sets fall within the FP32 margin (heavy ties), the
+                             search reruns in windows of work units that fit,
+                             with the same exact answer (result.fallback = 1) */
     int rebuild;          /* 1: recompute the single and pair count tables inside
                              this call, so their cost is part of it (the
                              reference counts them inside its search,
@@ -138,7 +148,8 @@ typedef struct {
     double fp32_max_abs_err;    /* max |FP32 - FP64| over re-scored candidates   */
     double fp32_max_rel_err;    /* same, relative to |FP64|                      */
     double margin;              /* FP32 candidate margin used (2 x error bound)  */
-    int fallback;               /* 1 if the exact fallback ran                   */
+    int fallback;               /* 1 if a candidate overflow made the fused path
+                                   rerun the search in windows                  */
     int tile;                   /* kernel tile id used                           */
     int seeded;                 /* 3-way fused: 1 if the search started from a
                                    bound found through the best pairs          */
@@ -194,14 +205,17 @@ typedef struct {
     int casesWords, controlsWords;
 } episdet_job;
 
-/* Runs many independent searches in one hipGraph launch: every 3-way job's
- * work units go through one persistent FP4-MFMA kernel launch, every 2-way
- * job's through one persistent VALU kernel launch, followed by one FP64
- * re-scoring kernel over all jobs' candidates.  Jobs the batch kernels cannot
- * take (PLAIN path, 3-way with >= 2^24 samples per class, candidate
- * overflow) run on their own afterwards.  opts[j] as for episdet_run (order
- * and score per job); batch_ms, if not NULL, receives the device time of the
- * graph.  Results are filled as episdet_run fills them. */
+/* Runs many independent searches in one hipGraph launch: the work units of
+ * every job go through one persistent FP4-MFMA kernel launch per kernel tile
+ * (AUTO jobs of one order share one tile), side by side in the graph,
+ * followed by one FP64 re-scoring kernel over all jobs' candidates.  Jobs the
+ * batch kernels cannot take (PLAIN and VALU paths, >= 2^24 samples per class,
+ * a second job on the same dataset, candidate overflow) run on their own
+ * afterwards.  opts[j] as for episdet_run (order and score per job);
+ * batch_ms, if not NULL, receives the device time of the graph.  Results are
+ * filled as episdet_run fills them; for jobs run in the graph, tile is the
+ * MFMA tile that ran, kernel_ms the graph's device time (re-scoring
+ * included) and rescore_ms 0. */
 int episdet_run_batch(episdet_dataset *const *ds, const episdet_options *opts, int njobs, episdet_result *results,
                       double *batch_ms);
 

@@ -276,6 +276,8 @@ double orc_score_mi(const uint32_t *cas, const uint32_t *ctl, int ncells, int Nc
 
 /* ----------------------------------------------------------------- search */
 
+uint64_t orc_table_hash(uint64_t rank, const uint32_t *cas, const uint32_t *ctl, int ncells);
+
 typedef struct {
     double score;       /* FP64 score of the winner                          */
     uint64_t packed;    /* X | Y<<21 | Z<<42, or A | B<<32                   */
@@ -423,9 +425,13 @@ static double score_tables(int order, int score, const uint32_t *cas, const uint
 /* Exhaustive search.  cases/controls: [n][2][Wc] / [n][2][Wt] u32 planes.
  * Only sets whose largest index lies in [last_lo, last_hi) are evaluated, so
  * a contiguous slice of the search can be timed; [0, n) is the full search.
- * nthreads <= 0 uses the OpenMP default.  Returns 0 on success. */
-int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, int Nt, int Wc, int Wt, int order,
-               int score, int nthreads, int last_lo, int last_hi, orc_result *out)
+ * nthreads <= 0 uses the OpenMP default.  checksum, when not NULL, receives
+ * the order-independent table checksum (orc_table_hash summed mod 2^64) of
+ * every evaluated set, the same quantity orc_all_tables returns, so a full
+ * search pins every table it scored as well as the winner.  Returns 0 on
+ * success. */
+int orc_search_ex(const uint32_t *cases, const uint32_t *controls, int n, int Nc, int Nt, int Wc, int Wt, int order,
+                  int score, int nthreads, int last_lo, int last_hi, orc_result *out, uint64_t *checksum)
 {
     if (order != 2 && order != 3) return 1;
     if (n < order || last_lo < 0 || last_hi > n || last_lo >= last_hi) return 1;
@@ -455,6 +461,8 @@ int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, i
     double best = score == ORC_K2 ? INFINITY : -INFINITY;
     uint64_t bestp = UINT64_MAX;
     int64_t sets = 0;
+    uint64_t hsum = 0;
+    const int want_hash = checksum != NULL;
 
     count8_fn count8 = pick_count8();
     #pragma omp parallel
@@ -462,6 +470,7 @@ int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, i
         double tb = score == ORC_K2 ? INFINITY : -INFINITY;
         uint64_t tp = UINT64_MAX;
         int64_t ts = 0;
+        uint64_t th = 0;
         int W64c = cx.W64[0], W64t = cx.W64[1];
         size_t qmax = (size_t)(W64c > W64t ? W64c : W64t);
         uint64_t *Q = (uint64_t *)malloc(sizeof(uint64_t) * 4 * qmax);
@@ -484,11 +493,13 @@ int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, i
                     }
                     double s = score_tables(2, score, cas, ctl, &cx, tab);
                     uint64_t p = (uint64_t)A | ((uint64_t)B << 32);
+                    if (want_hash) th += orc_table_hash((uint64_t)B * (B - 1) / 2 + (uint64_t)A, cas, ctl, 9);
                     if (better(score, s, p, tb, tp)) { tb = s; tp = p; }
                     ts++;
                 }
             } else {
                 int Z = hi;
+                const uint64_t rz3 = (uint64_t)Z * (Z - 1) * (Z - 2) / 6;   /* colex rank base C(Z,3) */
                 for (int Y = 1; Y < Z; Y++) {
                     /* Q[j*2+k] = Y_j & Z_k, per class, reused over every X < Y. */
                     uint32_t n8[2][64][8];
@@ -520,6 +531,8 @@ int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, i
                             triplet27_from8(n8[1][X - X0], xy, xz, yz[1], ctl);
                             double s = score_tables(3, score, cas, ctl, &cx, tab);
                             uint64_t p = (uint64_t)X | ((uint64_t)Y << 21) | ((uint64_t)Z << 42);
+                            if (want_hash)
+                                th += orc_table_hash(rz3 + (uint64_t)Y * (Y - 1) / 2 + (uint64_t)X, cas, ctl, 27);
                             if (better(score, s, p, tb, tp)) { tb = s; tp = p; }
                             ts++;
                         }
@@ -532,8 +545,10 @@ int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, i
         {
             if (better(score, tb, tp, best, bestp)) { best = tb; bestp = tp; }
             sets += ts;
+            hsum += th;
         }
     }
+    if (checksum) *checksum = hsum;
 
     out->score = best;
     out->packed = bestp;
@@ -553,6 +568,12 @@ int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, i
     free(cx.single);
     free(cx.pair);
     return 0;
+}
+
+int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, int Nt, int Wc, int Wt, int order,
+               int score, int nthreads, int last_lo, int last_hi, orc_result *out)
+{
+    return orc_search_ex(cases, controls, n, Nc, Nt, Wc, Wt, order, score, nthreads, last_lo, last_hi, out, NULL);
 }
 
 /* ---------------------------------------------------- full-table checking */

@@ -53,6 +53,8 @@ def lib():
         L.orc_search.argtypes = [_u32p, _u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_int, C.c_int, C.c_int, C.POINTER(OrcResult)]
         L.orc_search.restype = C.c_int
+        L.orc_search_ex.argtypes = L.orc_search.argtypes + [C.POINTER(C.c_uint64)]
+        L.orc_search_ex.restype = C.c_int
         L.orc_all_tables.argtypes = [_u32p, _u32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_void_p]
         L.orc_all_tables.restype = C.c_uint64
@@ -133,19 +135,25 @@ def score_mi(cas, ctl, table=None):
     return float(lib().orc_score_mi(cas, ctl, len(cas), Nc, Nt, table))
 
 
-def search(cases, controls, Nc, Nt, order, score, threads=0, last=None):
-    """Exhaustive search; returns dict(score, packed, sets, idx).  `last`
-    restricts the largest index of each set to [lo, hi)."""
+def search(cases, controls, Nc, Nt, order, score, threads=0, last=None, checksum=False):
+    """Exhaustive search; returns dict(score, packed, sets, idx), plus
+    "checksum" (the orc_table_hash sum of every evaluated set's table, as
+    all_tables computes it) when checksum=True.  `last` restricts the largest
+    index of each set to [lo, hi)."""
     cases = np.ascontiguousarray(cases, dtype=np.uint32)
     controls = np.ascontiguousarray(controls, dtype=np.uint32)
     n = cases.shape[0]
     lo, hi = last if last is not None else (0, n)
     r = OrcResult()
-    rc = lib().orc_search(cases, controls, n, Nc, Nt, cases.shape[2], controls.shape[2], order, score, threads,
-                          lo, hi, C.byref(r))
+    cs = C.c_uint64(0)
+    rc = lib().orc_search_ex(cases, controls, n, Nc, Nt, cases.shape[2], controls.shape[2], order, score, threads,
+                             lo, hi, C.byref(r), C.byref(cs) if checksum else None)
     if rc:
         raise RuntimeError("orc_search failed")
-    return {"score": r.score, "packed": int(r.packed), "sets": int(r.sets), "idx": tuple(r.idx)[:order]}
+    out = {"score": r.score, "packed": int(r.packed), "sets": int(r.sets), "idx": tuple(r.idx)[:order]}
+    if checksum:
+        out["checksum"] = int(cs.value)
+    return out
 
 
 def all_tables(cases, controls, Nc, Nt, order, want_tables=True):

@@ -79,10 +79,8 @@ int main(int argc, const char *argv[])
         return 1;
     }
     std::cout << "-------------------------------" << std::endl;
-    // The search is one persistent launch; the reference prints one line per
-    // outer block of its host loop (search-triplets.cu:164).
-    for (unsigned b = 0; b < numBlocks; b++)
-        std::cout << "Outer loop iteration " << b << " out of " << numBlocks << std::endl;
+    // The "Outer loop iteration" lines come from episdet_search itself, as
+    // they come from EpistasisDetectionSearch in the reference.
 
     int roundsCounter = 0;
     double searchTime = 0;

@@ -85,7 +85,9 @@ struct CandOut {
     unsigned *best_key;          // order-preserving key of the best FP32 badness
     unsigned long long *cand;    // packed indices of candidates
     float *cand_s32;             // their FP32 score
-    unsigned *cand_count;        // number appended (may exceed cand_cap: overflow)
+    unsigned *cand_count;        // [0]: number appended (may exceed cand_cap: overflow);
+                                 // [1]: sticky overflow flag, set by every append that
+                                 // finds no slot, since [0] can wrap past 2^32
     unsigned cand_cap;
     unsigned topk;               // > 1: keep every set that can be among the topk best
 };
@@ -156,6 +158,8 @@ __device__ void publish_topk(const CandOut &co, float margin, const float (&bad)
             if (slot < co.cand_cap) {
                 co.cand[slot] = packed[i];
                 co.cand_s32[slot] = bad[i];
+            } else {
+                co.cand_count[1] = 1u;
             }
         }
     }
@@ -199,6 +203,8 @@ __device__ __forceinline__ void publish_candidates(const CandOut &co, float marg
             if (slot < co.cand_cap) {
                 co.cand[slot] = packed[i];
                 co.cand_s32[slot] = bad[i];
+            } else {
+                co.cand_count[1] = 1u;
             }
         }
     }
@@ -619,74 +625,182 @@ __global__ __launch_bounds__(256) void k_triplets_valu(TriArgs p)
 
 // ==================================================== plain objective ====
 
+// 2-way plain round (search-pairs.cu:137-211): the four AND-counts of every
+// (A, B) of A block ab x B block bb (256 x 256 each), both classes, into
+// counts[2][256][256][4].  16 workgroups, one per 64 x 64 tile.  Rows up to
+// the next multiple of 256 exist in the device layout (zero past n).
+__global__ __launch_bounds__(256) void k_pairs_counts(const uint32_t *__restrict__ p0, int W0,
+                                                      const uint32_t *__restrict__ p1, int W1, int ab, int bb,
+                                                      uint32_t *__restrict__ counts)
+{
+    constexpr int R = 4, T = 16 * R;
+    __shared__ uint32_t sA[2][T][LDSW];
+    __shared__ uint32_t sB[2][T][LDSW];
+    const int tid = threadIdx.x, ta = tid >> 4, tb = tid & 15;
+    const int a0 = ab * 256 + (int)(blockIdx.x >> 2) * T, b0 = bb * 256 + (int)(blockIdx.x & 3) * T;
+    uint32_t acc0[R][R][4], acc1[R][R][4];
+    pair_count_class<R>(p0, W0, a0, b0, sA, sB, acc0);
+    pair_count_class<R>(p1, W1, a0, b0, sA, sB, acc1);
+#pragma unroll
+    for (int r = 0; r < R; r++)
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const int al = a0 - ab * 256 + ta + 16 * r, bl = b0 - bb * 256 + tb + 16 * q;
+            *(uint4 *)(counts + ((size_t)(0 * 256 + al) * 256 + bl) * 4) =
+                make_uint4(acc0[r][q][0], acc0[r][q][1], acc0[r][q][2], acc0[r][q][3]);
+            *(uint4 *)(counts + ((size_t)(1 * 256 + al) * 256 + bl) * 4) =
+                make_uint4(acc1[r][q][0], acc1[r][q][1], acc1[r][q][2], acc1[r][q][3]);
+        }
+}
+
 struct PlainObjArgs {
-    const uint32_t *counts;              // [2][256][256][8] from MODE_COUNTS
+    const uint32_t *counts;              // 3-way: [2][256][256][8] (MODE_COUNTS);
+                                         // 2-way: [2][256][256][4] (k_pairs_counts)
     const uint4 *singles;
-    const uint4 *pairs;
-    int n, N0, N1, score;
+    const uint4 *pairs;                  // 3-way only
+    int order, n, N0, N1, score;
     const double *t64;                   // FP64 table (ln m! or log2(m/N))
-    int X, yb, zb;
-    unsigned long long *best_key;        // key of the best FP64 badness
-    unsigned long long *cand;            // packed
-    double *cand_s64;
-    unsigned *cand_count;
-    unsigned cand_cap;
+    int X, yb, zb;                       // 3-way round: X, Y block, Z block; 2-way: A block yb, B block zb
+    unsigned long long *blk_out;         // [256][2] this round's block bests {key, packed}
+    const unsigned long long *blk_in;    // the previous round's on this stream, or null
+    unsigned long long *best;            // this stream's running best {key, packed}
+    unsigned *nan_flag;                  // set when a table is inconsistent (cells_valid)
 };
 
-// Derives all 27 cells from the round's count tables, scores in FP64, and
-// publishes every set whose exact score ties or beats the running best
-// (helper.cu:438-584, reshaped: one thread per (Y, Z), exact scores, no
-// nondeterministic index race).
-__global__ __launch_bounds__(256) void k_triplets_objective_plain(PlainObjArgs p)
+// (key, packed) pairs ordered by the FP64 badness key, then the packed index:
+// the tie rule of the exact search, lowest packed index among equal scores.
+__device__ __forceinline__ bool lex_less(unsigned long long k1, unsigned long long p1, unsigned long long k2,
+                                         unsigned long long p2)
 {
-    __shared__ double s_red[8];
+    return k1 < k2 || (k1 == k2 && p1 < p2);
+}
+__device__ __forceinline__ void wave_lexmin(unsigned long long &k, unsigned long long &pk)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const unsigned long long ok = __shfl_xor(k, o, 64), op = __shfl_xor(pk, o, 64);
+        if (lex_less(ok, op, k, pk)) {
+            k = ok;
+            pk = op;
+        }
+    }
+}
+
+// Folds a round's 256 block bests into a stream's running best (one wave).
+__device__ __forceinline__ void plain_fold(const unsigned long long *blk, unsigned long long *best)
+{
+    const int t = threadIdx.x;
+    unsigned long long k = ~0ull, pk = ~0ull;
+    for (int i = t; i < 256; i += 64) {
+        const unsigned long long kk = blk[2 * i], pp = blk[2 * i + 1];
+        if (lex_less(kk, pp, k, pk)) {
+            k = kk;
+            pk = pp;
+        }
+    }
+    wave_lexmin(k, pk);
+    if (t == 0 && lex_less(k, pk, best[0], best[1])) {
+        best[0] = k;
+        best[1] = pk;
+    }
+}
+
+// One round of the plain path: derives every cell of the round's sets from
+// its count tables, scores them in FP64 and keeps the round's best set by
+// (exact score, lowest packed index) -- the reference's objective kernel and
+// best-set reduction (helper.cu:438-584, 594-698, reduction.hpp:64-122)
+// with its nondeterministic index race replaced by a total order.  Workgroup
+// b holds Y (A) = yb*256 + b, thread t Z (B) = zb*256 + t.  Each workgroup
+// writes its best to blk_out; workgroup 0 first folds the previous round's
+// blk_in (that kernel ran before this one on the same stream) into the
+// stream's best, so rounds need no cross-workgroup synchronisation and no
+// candidate list that ties could overflow.
+__global__ __launch_bounds__(256) void k_objective_plain(PlainObjArgs p)
+{
+    __shared__ unsigned long long s_k[4], s_p[4];
     const int yl = blockIdx.x, zl = threadIdx.x;
-    const int X = p.X, Y = p.yb * 256 + yl, Z = p.zb * 256 + zl;
-    const bool valid = X < Y && Y < Z && Z < p.n;
+    if (yl == 0 && p.blk_in && zl < 64) plain_fold(p.blk_in, p.best);
     double bad = INFINITY;
+    bool valid;
+    unsigned long long packed;
+    if (p.order == 3) {
+        const int X = p.X, Y = p.yb * 256 + yl, Z = p.zb * 256 + zl;
+        valid = X < Y && Y < Z && Z < p.n;
+        packed = (unsigned long long)X | ((unsigned long long)Y << 21) | ((unsigned long long)Z << 42);
+        if (valid) {
+            const uint32_t *c0 = p.counts + ((size_t)(0 * 256 + yl) * 256 + zl) * 8;
+            const uint32_t *c1 = p.counts + ((size_t)(1 * 256 + yl) * 256 + zl) * 8;
+            uint32_t n8c[8], n8t[8];
+            for (int i = 0; i < 8; i++) {
+                n8c[i] = c0[i];
+                n8t[i] = c1[i];
+            }
+            const Singles sx = ld_global(p.singles, X), sy = ld_global(p.singles, Y), sz = ld_global(p.singles, Z);
+            uint32_t xy[9], xz[9], yz[9], cas[27], ctl[27];
+            pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 0, (uint32_t)p.N0, xy);
+            pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 0, (uint32_t)p.N0, xz);
+            pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 0, (uint32_t)p.N0, yz);
+            triplet27(n8c, xy, xz, yz, cas);
+            pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 1, (uint32_t)p.N1, xy);
+            pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 1, (uint32_t)p.N1, xz);
+            pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 1, (uint32_t)p.N1, yz);
+            triplet27(n8t, xy, xz, yz, ctl);
+            bad = p.score == SCORE_K2 ? k2_f64(cas, ctl, 27, p.N0, p.N1, p.t64)
+                                      : -mi_f64(cas, ctl, 27, p.N0, p.N1, p.t64);
+        }
+    } else {
+        const int A = p.yb * 256 + yl, B = p.zb * 256 + zl;
+        valid = A < B && B < p.n;
+        packed = (unsigned long long)A | ((unsigned long long)B << 32);
+        if (valid) {
+            const uint4 q0 = *(const uint4 *)(p.counts + ((size_t)(0 * 256 + yl) * 256 + zl) * 4);
+            const uint4 q1 = *(const uint4 *)(p.counts + ((size_t)(1 * 256 + yl) * 256 + zl) * 4);
+            const Singles sa = ld_global(p.singles, A), sb = ld_global(p.singles, B);
+            uint32_t cas[9], ctl[9];
+            pair9(q0.x, q0.y, q0.z, q0.w, sa.x, sa.y, sb.x, sb.y, (uint32_t)p.N0, cas);
+            pair9(q1.x, q1.y, q1.z, q1.w, sa.z, sa.w, sb.z, sb.w, (uint32_t)p.N1, ctl);
+            bad = p.score == SCORE_K2 ? k2_f64(cas, ctl, 9, p.N0, p.N1, p.t64) : -mi_f64(cas, ctl, 9, p.N0, p.N1, p.t64);
+        }
+    }
+    unsigned long long key = ~0ull, pk = ~0ull;
     if (valid) {
-        const uint32_t *c0 = p.counts + ((size_t)(0 * 256 + yl) * 256 + zl) * 8;
-        const uint32_t *c1 = p.counts + ((size_t)(1 * 256 + yl) * 256 + zl) * 8;
-        uint32_t n8c[8], n8t[8];
-        for (int i = 0; i < 8; i++) {
-            n8c[i] = c0[i];
-            n8t[i] = c1[i];
+        if (bad != bad) *p.nan_flag = 1u;         // inconsistent tables: reported, never ranked
+        else {
+            key = d2key(bad);
+            pk = packed;
         }
-        const Singles sx = ld_global(p.singles, X), sy = ld_global(p.singles, Y), sz = ld_global(p.singles, Z);
-        uint32_t xy[9], xz[9], yz[9], cas[27], ctl[27];
-        pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 0, (uint32_t)p.N0, xy);
-        pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 0, (uint32_t)p.N0, xz);
-        pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 0, (uint32_t)p.N0, yz);
-        triplet27(n8c, xy, xz, yz, cas);
-        pair9_lookup(p.pairs, p.n, sx, sy, X, Y, 1, (uint32_t)p.N1, xy);
-        pair9_lookup(p.pairs, p.n, sx, sz, X, Z, 1, (uint32_t)p.N1, xz);
-        pair9_lookup(p.pairs, p.n, sy, sz, Y, Z, 1, (uint32_t)p.N1, yz);
-        triplet27(n8t, xy, xz, yz, ctl);
-        bad = p.score == SCORE_K2 ? k2_f64(cas, ctl, 27, p.N0, p.N1, p.t64) : -mi_f64(cas, ctl, 27, p.N0, p.N1, p.t64);
     }
-    const double wmin = wave_min_d(bad);
-    const int tid = threadIdx.x;
-    if ((tid & 63) == 0) s_red[tid >> 6] = wmin;
-    __syncthreads();
-    if (tid == 0) {
-        double b = fmin(fmin(s_red[0], s_red[1]), fmin(s_red[2], s_red[3]));
-        double thr = -INFINITY;
-        if (b < INFINITY) {
-            unsigned long long old = atomicMin(p.best_key, d2key(b));
-            thr = fmin(key2d(old), b);
-        }
-        s_red[4] = thr;
+    wave_lexmin(key, pk);
+    const int w = zl >> 6;
+    if ((zl & 63) == 0) {
+        s_k[w] = key;
+        s_p[w] = pk;
     }
     __syncthreads();
-    // a NaN score (inconsistent tables, see cells_valid) is published too,
-    // so that the host reports it instead of skipping the set
-    if (valid && (bad <= s_red[4] || bad != bad)) {
-        unsigned slot = atomicAdd(p.cand_count, 1u);
-        if (slot < p.cand_cap) {
-            p.cand[slot] = (unsigned long long)X | ((unsigned long long)Y << 21) | ((unsigned long long)Z << 42);
-            p.cand_s64[slot] = bad;
-        }
+    if (zl == 0) {
+        unsigned long long k = s_k[0], q = s_p[0];
+        for (int i = 1; i < 4; i++)
+            if (lex_less(s_k[i], s_p[i], k, q)) {
+                k = s_k[i];
+                q = s_p[i];
+            }
+        p.blk_out[2 * yl] = k;
+        p.blk_out[2 * yl + 1] = q;
     }
+}
+
+// Folds a stream's last round into its best (one wave, launched on that
+// stream after its last objective kernel).
+__global__ __launch_bounds__(64) void k_plain_fold(const unsigned long long *blk, unsigned long long *best)
+{
+    plain_fold(blk, best);
+}
+
+// Fills n 64-bit words with v (no memset node, so a captured sequence stays
+// replay-safe).
+__global__ void k_fill_u64(unsigned long long *dst, int n, unsigned long long v)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = v;
 }
 
 // ======================================================= FP64 rescoring ==
@@ -787,7 +901,8 @@ __global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)
 // are read on the device, so the whole batch needs no host round trip.
 struct RescoreJob {
     RescoreArgs r;                 // r.count unused
-    const unsigned *count;         // candidates appended by the search
+    const unsigned *count;         // candidates appended by the search, and the
+                                   // overflow flag (CandOut::cand_count)
     unsigned cap;                  // buffer capacity (larger counts: overflow)
 };
 
@@ -797,8 +912,8 @@ __global__ __launch_bounds__(64) void k_rescore_batch(const RescoreJob *jobs, in
         unsigned long long start = 0;
         int j = 0;
         for (; j < njobs; j++) {
-            const unsigned c = *jobs[j].count;
-            const unsigned long long cnt = c > jobs[j].cap ? 0ull : (unsigned long long)c;
+            const unsigned c = jobs[j].count[0];
+            const unsigned long long cnt = (c > jobs[j].cap || jobs[j].count[1]) ? 0ull : (unsigned long long)c;
             if (gi < start + cnt) break;
             start += cnt;
         }
@@ -815,7 +930,8 @@ __global__ void k_init_many(unsigned long long *const *queue, unsigned *const *b
     if (i < n) {
         *queue[i] = 0ull;
         *best_key[i] = 0xFF800000u;
-        *cand_count[i] = 0u;
+        cand_count[i][0] = 0u;
+        cand_count[i][1] = 0u;
     }
     if (i < nq) batch_queues[i] = 0ull;
 }
@@ -855,7 +971,8 @@ __global__ void k_init_search(unsigned long long *queue, unsigned *best_key, uns
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         *queue = 0ull;
         *best_key = 0xFF800000u;                 // key of +inf
-        *cand_count = 0u;
+        cand_count[0] = 0u;
+        cand_count[1] = 0u;                      // overflow flag
         *best_key64 = 0xFFF0000000000000ull;     // key of +inf (double)
         *checksum = 0ull;
     }

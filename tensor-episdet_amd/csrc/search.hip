@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 #include <tuple>
@@ -72,9 +73,12 @@ struct Ctl {                  // device control block
     unsigned long long best64;
     unsigned long long checksum;
     unsigned best32;
-    unsigned cand_count;
+    unsigned cand_count;      // candidates appended; with cand_over, CandOut::cand_count[0..1]
+    unsigned cand_over;       // sticky overflow flag (the count can wrap past 2^32)
     unsigned bad_snp;         // lowest SNP with a sample in both planes (k_singles), or ~0
 };
+static_assert(offsetof(Ctl, cand_over) == offsetof(Ctl, cand_count) + sizeof(unsigned),
+              "the overflow flag follows the candidate count");
 
 }  // namespace
 
@@ -278,6 +282,62 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
     return sc;
 }
 
+struct DevList {   // device allocations released together
+    std::vector<void *> ptrs;
+    ~DevList()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+    template <typename T>
+    int alloc(T **dst, size_t n)
+    {
+        *dst = nullptr;
+        if (hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return EPISDET_ERR_ALLOC;
+        ptrs.push_back((void *)*dst);
+        return EPISDET_OK;
+    }
+    template <typename T>
+    int put(T **dst, const T *src, size_t n)
+    {
+        CHK(alloc(dst, n));
+        if (n && hipMemcpy((void *)*dst, (const void *)src, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess)
+            return EPISDET_ERR_HIP;
+        return EPISDET_OK;
+    }
+};
+
+// Streams (default priority) with one event each, destroyed together.
+struct StreamSet {
+    std::vector<hipStream_t> st;
+    std::vector<hipEvent_t> ev;
+    int create(int k)
+    {
+        for (int i = 0; i < k; i++) {
+            hipStream_t q;
+            HIPCHK(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+            st.push_back(q);
+            hipEvent_t e;
+            HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ev.push_back(e);
+        }
+        return EPISDET_OK;
+    }
+    ~StreamSet()
+    {
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        for (hipStream_t q : st) (void)hipStreamDestroy(q);
+    }
+};
+
+// Inverse of the kernels' order-preserving double key (d2key).
+double key2d_host(unsigned long long k)
+{
+    const unsigned long long u = (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFull) : ~k;
+    double d;
+    std::memcpy(&d, &u, sizeof d);
+    return d;
+}
+
 int ensure_pairs(episdet_dataset *ds);
 
 int launch_blocks(const void *kernel, int cu)
@@ -359,14 +419,16 @@ int upload_single(episdet_dataset *ds, const A &a, unsigned long long units)
     return EPISDET_OK;
 }
 
+// upload = false: the caller has already copied `a` to the dataset's
+// argument slot (upload_single), e.g. before recording a timing event.
 template <int MODE>
-int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s)
+int dispatch_tri_mfma(episdet_dataset *ds, int tile, TriArgs &a, hipStream_t s, bool upload = true)
 {
     const void *k = mfma_kernel<MODE>(tile);
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k, mfma_threads(tile), 0) != hipSuccess || per < 1)
         per = 1;
-    CHK(upload_single(ds, a, a.nunits));
+    if (upload) CHK(upload_single(ds, a, a.nunits));
     BatchTri bt;
     bt.jobs = (const TriArgs *)ds->jobargs;
     bt.job_start = ds->jobstart;
@@ -438,9 +500,9 @@ int launch_pairs_mfma(int tile, int blocks, const BatchPair &bp, hipStream_t s)
 void fill_pairs(episdet_dataset *ds, PairArgs &a, bool mfma, int tile);
 
 template <int MODE>
-int dispatch_pairs(episdet_dataset *ds, bool mfma, int tile, PairArgs &a, hipStream_t s)
+int dispatch_pairs(episdet_dataset *ds, bool mfma, int tile, PairArgs &a, hipStream_t s, bool upload = true)
 {
-    CHK(upload_single(ds, a, a.nunits));
+    if (upload) CHK(upload_single(ds, a, a.nunits));
     BatchPair bp;
     bp.jobs = (const PairArgs *)ds->jobargs;
     bp.job_start = ds->jobstart;
@@ -632,67 +694,89 @@ int rescore(episdet_dataset *ds, int order, int score, int count, double *ms)
     return EPISDET_OK;
 }
 
-int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res);
+// Re-scored candidates gathered over one or more launches: the winner (FP64
+// score, then the lowest packed index), the FP32-vs-FP64 error over every
+// candidate, and for top-k the best topk so far.
+struct Collected {
+    int score = 0, topk = 0;
+    bool have = false;
+    double bs = 0;                  // FP64 score of the winner
+    unsigned long long bp = ~0ull;  // its packed index
+    float b32 = 0.f;                // its FP32 score
+    double maxabs = 0, maxrel = 0;
+    long long count = 0;
+    std::vector<std::pair<double, unsigned long long>> top;   // better() order, at most topk
+};
 
-// Winner among the `count` re-scored candidates on the device (FP64 score,
-// then the lowest packed index), and the FP32-vs-FP64 error over them.
-int collect_best(episdet_dataset *ds, int order, int score, int count, episdet_result *res)
+// Adds the `count` candidates re-scored on the device (ds->cand,
+// ds->cand_s32, ds->cand_s64) to acc.
+int collect(episdet_dataset *ds, int count, Collected &acc)
 {
+    if (count <= 0) return EPISDET_OK;
+    const int score = acc.score;
     std::vector<unsigned long long> hp(count);
     std::vector<float> h32(count);
     std::vector<double> h64(count);
     HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(h32.data(), ds->cand_s32, sizeof(float) * count, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
-    double bs = score == EPISDET_SCORE_K2 ? INFINITY : -INFINITY;
-    unsigned long long bp = ~0ull;
-    float b32 = 0.f;
-    double maxabs = 0, maxrel = 0;
     for (int i = 0; i < count; i++) {
         if (std::isnan(h64[i])) return EPISDET_ERR_INTERNAL;   // inconsistent count tables
         const double s32 = score == EPISDET_SCORE_K2 ? (double)h32[i] : -(double)h32[i];
         const double err = std::fabs(s32 - h64[i]);
-        maxabs = std::max(maxabs, err);
-        if (h64[i] != 0) maxrel = std::max(maxrel, err / std::fabs(h64[i]));
-        if (better(score, h64[i], hp[i], bs, bp)) {
-            bs = h64[i];
-            bp = hp[i];
-            b32 = (float)s32;
+        acc.maxabs = std::max(acc.maxabs, err);
+        if (h64[i] != 0) acc.maxrel = std::max(acc.maxrel, err / std::fabs(h64[i]));
+        if (!acc.have || better(score, h64[i], hp[i], acc.bs, acc.bp)) {
+            acc.have = true;
+            acc.bs = h64[i];
+            acc.bp = hp[i];
+            acc.b32 = (float)s32;
         }
     }
-    res->best_score = bs;
-    res->best_score_f32 = b32;
-    res->packed = bp;
-    unpack(order, bp, res->idx);
-    res->fp32_max_abs_err = maxabs;
-    res->fp32_max_rel_err = maxrel;
-    res->candidates = count;
-    return EPISDET_OK;
-}
-
-// The k best candidates by FP64 score (better() order), best first.
-int collect_topk(episdet_dataset *ds, int order, int score, int count, int k, episdet_result *res)
-{
-    std::vector<unsigned long long> hp(count);
-    std::vector<double> h64(count);
-    HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
-    std::vector<int> ord(count);
-    for (int i = 0; i < count; i++) ord[i] = i;
-    const int m = std::min(k, count);
-    std::partial_sort(ord.begin(), ord.begin() + m, ord.end(),
-                      [&](int a, int b) { return better(score, h64[a], hp[a], h64[b], hp[b]); });
-    ds->topk_packed.resize(m);
-    ds->topk_scores.resize(m);
-    for (int i = 0; i < m; i++) {
-        ds->topk_packed[i] = hp[ord[i]];
-        ds->topk_scores[i] = h64[ord[i]];
+    if (acc.topk > 1) {
+        for (int i = 0; i < count; i++) acc.top.emplace_back(h64[i], hp[i]);
+        const size_t m = std::min<size_t>((size_t)acc.topk, acc.top.size());
+        std::partial_sort(acc.top.begin(), acc.top.begin() + m, acc.top.end(),
+                          [&](const std::pair<double, unsigned long long> &a,
+                              const std::pair<double, unsigned long long> &b) {
+                              return better(score, a.first, a.second, b.first, b.second);
+                          });
+        acc.top.resize(m);
     }
-    (void)order;
-    (void)res;
+    acc.count += count;
     return EPISDET_OK;
 }
 
+// Writes the gathered winner (and, for top-k, the ranking) into res and ds.
+int finish(episdet_dataset *ds, int order, const Collected &acc, episdet_result *res)
+{
+    if (!acc.have) return EPISDET_ERR_INTERNAL;   // the winner is always a candidate
+    res->best_score = acc.bs;
+    res->best_score_f32 = acc.b32;
+    res->packed = acc.bp;
+    unpack(order, acc.bp, res->idx);
+    res->fp32_max_abs_err = acc.maxabs;
+    res->fp32_max_rel_err = acc.maxrel;
+    res->candidates = acc.count;
+    if (acc.topk > 1) {
+        ds->topk_packed.resize(acc.top.size());
+        ds->topk_scores.resize(acc.top.size());
+        for (size_t i = 0; i < acc.top.size(); i++) {
+            ds->topk_packed[i] = acc.top[i].second;
+            ds->topk_scores[i] = acc.top[i].first;
+        }
+    }
+    return EPISDET_OK;
+}
+
+// Winner of `count` candidates already re-scored on the device.
+int collect_best(episdet_dataset *ds, int order, int score, int count, episdet_result *res)
+{
+    Collected acc;
+    acc.score = score;
+    CHK(collect(ds, count, acc));
+    return finish(ds, order, acc, res);
+}
 
 // One fused kernel configuration for one dataset and search: the argument
 // block and, for 3-way, the work-unit groups (host copy kept for counting
@@ -767,23 +851,52 @@ int plan_fused(episdet_dataset *ds, int order, int score, int path, int tile, Fu
     return EPISDET_OK;
 }
 
-// Launches the plan over work units [u0, u1) (the whole search is
-// [0, nunits)) and returns the kernel time; the control block is reset and
-// the queue starts at u0.
-int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long long u0, unsigned long long u1,
-                 float *ms)
+// Most sets one work unit of a plan can append: a candidate buffer at least
+// this large always holds a launch over any one unit.
+unsigned long long unit_sets_max(const FusedPlan &pl)
 {
+    if (pl.order == 3) {
+        if (pl.mfma) return (unsigned long long)mfma_xstep(pl.tile) * mfma_ty(pl.tile) * mfma_tz(pl.tile);
+        const Tri3Tile t = TRI_TILES[pl.tile];
+        return 256ull * t.A * t.B;
+    }
+    if (pl.mfma) return pl.tile == 0 ? (unsigned long long)P2_TA * P2_TB : (unsigned long long)P3_TA * P3_TB;
+    return 256ull * PAIR_TILES[pl.tile] * PAIR_TILES[pl.tile];
+}
+
+// Launches the plan over work units [u0, u1) (the whole search is
+// [0, nunits)) with `cap` candidate slots and the global bound starting at
+// seed_key (0: +inf), and returns the kernel time.  The argument block is
+// uploaded before the first timing event, so the time covers the search
+// kernel alone.
+int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long long u0, unsigned long long u1,
+                 unsigned cap, unsigned seed_key, float *ms)
+{
+    if (cap > ds->cand_cap) return EPISDET_ERR_INTERNAL;
     TriArgs &ta = pl.ta;
     PairArgs &pa = pl.pa;
     ta.co.cand = pa.co.cand = ds->cand;
     ta.co.cand_s32 = pa.co.cand_s32 = ds->cand_s32;
-    ta.co.cand_cap = pa.co.cand_cap = ds->cand_cap;
+    ta.co.cand_cap = pa.co.cand_cap = cap;
     ta.co.topk = pa.co.topk = (unsigned)topk;
+    // the launch covers units [u0, u1): the queue starts at u0 and the
+    // kernels stop at nunits (restored below)
     ta.nunits = pa.nunits = u1;
+    struct Restore {
+        FusedPlan &pl;
+        ~Restore() { pl.ta.nunits = pl.pa.nunits = pl.nunits; }
+    } restore{pl};
+    const int order = pl.order, tile = pl.tile;
+    const bool mfma = pl.mfma;
+    int rc = EPISDET_OK;
+    // the batched kernels read their argument block from device memory (the
+    // VALU 3-way kernel takes it by value)
+    if (order == 3 && mfma) CHK(upload_single(ds, ta, ta.nunits));
+    else if (order == 2) CHK(upload_single(ds, pa, pa.nunits));
     CHK(init_ctl(ds));
     if (u0) HIPCHK(hipMemcpyAsync(&ds->ctl->queue, &u0, sizeof u0, hipMemcpyHostToDevice, ds->stream));
-    if (pl.seed_key)
-        HIPCHK(hipMemcpyAsync(&ds->ctl->best32, &pl.seed_key, sizeof pl.seed_key, hipMemcpyHostToDevice, ds->stream));
+    if (seed_key)
+        HIPCHK(hipMemcpyAsync(&ds->ctl->best32, &seed_key, sizeof seed_key, hipMemcpyHostToDevice, ds->stream));
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     if (hipEventCreate(&e1) != hipSuccess) {
@@ -791,24 +904,20 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
         return EPISDET_ERR_HIP;
     }
     (void)hipEventRecord(e0, ds->stream);
-    const int order = pl.order, tile = pl.tile;
-    const bool mfma = pl.mfma;
-    int rc;
     if (topk)
-        rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_TOPK>(ds, tile, ta, ds->stream)
+        rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_TOPK>(ds, tile, ta, ds->stream, false)
                                 : dispatch_tri<MODE_TOPK>(ds, tile, ta, 0, ds->stream))
-                        : dispatch_pairs<MODE_TOPK>(ds, mfma, tile, pa, ds->stream);
+                        : dispatch_pairs<MODE_TOPK>(ds, mfma, tile, pa, ds->stream, false);
     else
-        rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream)
+        rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream, false)
                                 : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
-                        : dispatch_pairs<MODE_SEARCH>(ds, mfma, tile, pa, ds->stream);
+                        : dispatch_pairs<MODE_SEARCH>(ds, mfma, tile, pa, ds->stream, false);
     (void)hipEventRecord(e1, ds->stream);
     const bool synced = hipEventSynchronize(e1) == hipSuccess;
     float k = 0;
     if (synced) (void)hipEventElapsedTime(&k, e0, e1);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    ta.nunits = pa.nunits = pl.nunits;
     if (rc == EPISDET_OK && !synced) rc = EPISDET_ERR_HIP;
     *ms = k;
     return rc;
@@ -1024,8 +1133,8 @@ int tune_kernel(episdet_dataset *ds, int order, int score, std::vector<episdet_t
         const unsigned long long u0 = (pl.nunits - k) / 2;
         const unsigned long long sets = count_sets(ds, pl, u0, u0 + k);
         float ms = 0;
-        CHK(launch_fused(ds, pl, 0, u0, u0 + k, &ms));    // first launch loads the code object
-        CHK(launch_fused(ds, pl, 0, u0, u0 + k, &ms));
+        CHK(launch_fused(ds, pl, 0, u0, u0 + k, ds->cand_cap, 0, &ms));   // first launch loads the code object
+        CHK(launch_fused(ds, pl, 0, u0, u0 + k, ds->cand_cap, 0, &ms));
         const double est = sets ? (double)ms * all_sets / (double)sets : INFINITY;
         if (table) {
             episdet_tune_entry e;
@@ -1086,6 +1195,66 @@ int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *til
     return EPISDET_OK;
 }
 
+// Key of the smallest float at or above badness + margin / 2, or 0 when
+// that is not finite.  A launch seeded with it keeps as candidates every set
+// whose exact badness is at or below `badness` (each FP32 badness is within
+// half the margin of the exact one), as seed_from_pairs argues.
+unsigned seed_key_above(double badness, float margin)
+{
+    const double bound = badness + 0.5 * (double)margin;
+    float f = (float)bound;
+    if ((double)f < bound) f = std::nextafter(f, INFINITY);
+    if (!std::isfinite(f)) return 0u;
+    return f2key_host(f);
+}
+
+// Exact answer after a candidate overflow (heavy ties, degenerate data): the
+// work queue is searched again in consecutive unit windows, each re-scored
+// in FP64 and gathered before the next, so no window holds more candidates
+// than the buffer.  A window of cap / unit_sets_max(pl) units can never
+// overflow; windows start larger when the first launch's count allows and
+// double while they stay well below the cap.  Each window's global bound
+// starts at the best exact badness gathered so far (the k-th best for
+// top-k) plus half the margin, which keeps every set that can still win.
+int run_sliced(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned cap, const Ctl &first, Collected &acc,
+               double *kms_total, double *rms_total)
+{
+    const unsigned long long floor_step = std::max<unsigned long long>(1ull, cap / unit_sets_max(pl));
+    unsigned long long step = floor_step;
+    // the first launch appended at least cand_count sets when that exceeds
+    // the cap (more if the count wrapped, which the shrinking below absorbs)
+    if (first.cand_count > cap)
+        step = std::max(step, (unsigned long long)((double)pl.nunits * (double)cap / (2.0 * first.cand_count)));
+    const float margin = pl.sc.margin;
+    for (unsigned long long u = 0; u < pl.nunits;) {
+        const unsigned long long u1 = std::min(pl.nunits, u + step);
+        unsigned seed = pl.seed_key;
+        const bool bounded = topk ? (int)acc.top.size() >= topk : acc.have;
+        if (bounded) {
+            const double s = topk ? acc.top.back().first : acc.bs;
+            const unsigned k = seed_key_above(pl.score == EPISDET_SCORE_K2 ? s : -s, margin);
+            if (k && (!seed || k < seed)) seed = k;
+        }
+        float kms = 0;
+        CHK(launch_fused(ds, pl, topk, u, u1, cap, seed, &kms));
+        *kms_total += kms;
+        Ctl hc;
+        HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
+        if (hc.cand_over || hc.cand_count > cap) {
+            if (step <= floor_step) return EPISDET_ERR_INTERNAL;   // cannot happen: see above
+            step = std::max(floor_step, step / 4);
+            continue;
+        }
+        double rms = 0;
+        CHK(rescore(ds, pl.order, pl.score, (int)hc.cand_count, &rms));
+        *rms_total += rms;
+        CHK(collect(ds, (int)hc.cand_count, acc));
+        u = u1;
+        if (hc.cand_count < cap / 8) step *= 2;
+    }
+    return EPISDET_OK;
+}
+
 // Fused search: one persistent launch, then FP64 re-scoring of the
 // candidates.  With o->topk > 1 the candidates cover the topk best sets.
 int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
@@ -1097,13 +1266,17 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     CHK(ensure_tables(ds, score));
     if (order == 3) CHK(ensure_pairs(ds));
     if (path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, order, score, &path, &tile, o->autotune != 0));
-    unsigned cap = o->candidate_cap > 0 ? (unsigned)o->candidate_cap : DEFAULT_CAP;
-    // top-k: before the bound settles, each resident block can append about
-    // 2k sets of its first units
-    if (topk) cap = std::max<unsigned>(cap, (unsigned)std::min<unsigned long long>(1ull << 26, 32ull * topk * ds->cu));
-    CHK(ensure_cand(ds, cap));
     FusedPlan pl;
     CHK(plan_fused(ds, order, score, path, tile, pl));
+    // Candidate slots of this search: the requested capacity, by default 1M
+    // and for top-k at least what the resident blocks append before the
+    // bound settles (about 2k sets each); never fewer than one work unit's
+    // sets, which is all a window of the sliced rerun needs.
+    unsigned long long cap = DEFAULT_CAP;
+    if (o->candidate_cap > 0) cap = (unsigned long long)o->candidate_cap;
+    else if (topk) cap = std::max<unsigned long long>(cap, std::min<unsigned long long>(1ull << 26, 32ull * topk * ds->cu));
+    cap = std::max(cap, unit_sets_max(pl));
+    CHK(ensure_cand(ds, (unsigned)cap));
     const ScoreCtx &sc = pl.sc;
     if (order == 3 && !topk) {
         unsigned key = 0;
@@ -1111,199 +1284,172 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     }
     res->seeded = pl.seed_key != 0;
 
-    // One search launch; with top-k, a candidate overflow grows the buffer
-    // and launches again (at most twice).
+    Collected acc;
+    acc.score = score;
+    acc.topk = topk;
+    float kms = 0;
+    CHK(launch_fused(ds, pl, topk, 0, pl.nunits, (unsigned)cap, pl.seed_key, &kms));
+    double kms_total = kms, rms_total = 0;
     Ctl hc;
-    int rc = EPISDET_OK;
-    double kms_total = 0;
-    for (int attempt = 0;; attempt++) {
-        float kms = 0;
-        rc = launch_fused(ds, pl, topk, 0, pl.nunits, &kms);
-        if (rc != EPISDET_OK) break;
-        kms_total += kms;
-        if (hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost) != hipSuccess) {
-            rc = EPISDET_ERR_HIP;
-            break;
-        }
-        if (!topk || hc.cand_count <= ds->cand_cap || attempt == 2) break;
-        const unsigned long long grow = std::min<unsigned long long>(1ull << 28, 4ull * hc.cand_count);
-        if (grow <= ds->cand_cap) break;
-        rc = ensure_cand(ds, (unsigned)grow);
-        if (rc != EPISDET_OK) break;
+    HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
+    if (!hc.cand_over && hc.cand_count <= cap) {
+        CHK(rescore(ds, order, score, (int)hc.cand_count, &rms_total));
+        CHK(collect(ds, (int)hc.cand_count, acc));
+    } else {
+        // more sets within the FP32 margin than slots: search again in windows
+        CHK(run_sliced(ds, pl, topk, (unsigned)cap, hc, acc, &kms_total, &rms_total));
+        res->fallback = 1;
     }
-    if (rc != EPISDET_OK) return rc;
     res->kernel_ms = kms_total;
+    res->rescore_ms = rms_total;
     res->margin = sc.margin;
     res->tile = tile;
-    res->candidates = hc.cand_count;
-    if (hc.cand_count == 0) return EPISDET_ERR_INTERNAL;
-    if (hc.cand_count > ds->cand_cap) {
-        if (topk) return EPISDET_ERR_CANDIDATES;
-        // Too many sets within the FP32 margin (heavy ties, degenerate data):
-        // answer exactly with the FP64 round loop instead (3-way).
-        if (order != 3) return EPISDET_ERR_CANDIDATES;
-        episdet_options po = *o;
-        po.path = EPISDET_PATH_PLAIN;
-        po.plain_rounds = 0;
-        const double k = res->kernel_ms;
-        CHK(run_plain(ds, &po, res));
-        res->kernel_ms += k;
-        res->fallback = 1;
-        res->path = path;
-        return EPISDET_OK;
-    }
-    const int count = (int)hc.cand_count;
-    double rms = 0;
-    CHK(rescore(ds, order, score, count, &rms));
-    res->rescore_ms = rms;
-    CHK(collect_best(ds, order, score, count, res));
-    if (topk) CHK(collect_topk(ds, order, score, count, topk, res));
+    CHK(finish(ds, order, acc, res));
     res->path = path;
     return EPISDET_OK;
 }
 
-// Reference-shaped 3-way path: host loop over (Y block, X, Z block) rounds
-// of 256 (search-triplets.cu:162-273) on NUM_STREAMS streams; each round is a
-// count kernel writing int32 tables to HBM and an FP64 objective kernel.
-// 2-way plain reuses the fused 2-way counting with FP64 candidates.
+// Reference-shaped path (search-pairs.cu:137-211, search-triplets.cu:162-273):
+// a host loop of rounds of 256 on NUM_STREAMS streams, each a count kernel
+// writing the round's int32 AND-count tables to HBM and an FP64 objective
+// kernel that derives, scores and keeps the round's best set.  3-way rounds
+// are (Y block, X, Z block >= Y block); 2-way rounds are block pairs
+// (A block <= B block).  Each stream keeps its best (exact score, lowest
+// packed index); the host takes the best of the streams, so ties of any
+// size give the same exact answer.
 int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res)
 {
     const int order = o->order, score = o->score;
     CHK(ensure_tables(ds, score));
-    CHK(ensure_cand(ds, o->candidate_cap > 0 ? (unsigned)o->candidate_cap : DEFAULT_CAP));
-    if (order == 2) {
-        // There is no separate 2-way round loop: PLAIN for pairs runs the
-        // fused 2-way kernel (its winner is exact after FP64 re-scoring).
-        episdet_options fo = *o;
-        fo.path = EPISDET_PATH_FUSED_VALU;
-        return run_fused(ds, &fo, res);
-    }
-    CHK(ensure_pairs(ds));
+    if (order == 3) CHK(ensure_pairs(ds));
     const int n = ds->n, nb = (n + BLOCK - 1) / BLOCK;
+    const size_t slot = (size_t)2 * 256 * 256 * (order == 3 ? 8 : 4);
+    DevList dl;
     uint32_t *counts = nullptr;
-    const size_t slot = (size_t)2 * 256 * 256 * 8;
-    if (hipMalloc(&counts, sizeof(uint32_t) * slot * NUM_STREAMS) != hipSuccess) return EPISDET_ERR_ALLOC;
-    hipStream_t st[NUM_STREAMS];
-    for (int i = 0; i < NUM_STREAMS; i++) HIPCHK(hipStreamCreateWithFlags(&st[i], hipStreamNonBlocking));
-    CHK(init_ctl(ds));
+    unsigned long long *blk = nullptr, *sbest = nullptr, *flag = nullptr;
+    CHK(dl.alloc(&counts, slot * NUM_STREAMS));
+    CHK(dl.alloc(&blk, (size_t)NUM_STREAMS * 2 * 512));      // [stream][parity][256][2]
+    CHK(dl.alloc(&sbest, (size_t)NUM_STREAMS * 2));          // [stream][2]
+    CHK(dl.alloc(&flag, 1));
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ds->stream, sbest, NUM_STREAMS * 2, ~0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, ds->stream, flag, 1, 0ull);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ds->stream));
-    hipEvent_t e0, e1, ev[NUM_STREAMS];
+    StreamSet ss;
+    CHK(ss.create(NUM_STREAMS));
+    hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    for (int i = 0; i < NUM_STREAMS; i++) HIPCHK(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
     HIPCHK(hipEventRecord(e0, ds->stream));
-    for (int i = 0; i < NUM_STREAMS; i++) HIPCHK(hipStreamWaitEvent(st[i], e0, 0));
+    for (int i = 0; i < NUM_STREAMS; i++) HIPCHK(hipStreamWaitEvent(ss.st[i], e0, 0));
 
     const int plain_tile = 1;   // 32 x 32 (Y, Z) per workgroup: 64 workgroups per round
     TriArgs ta;
-    fill_tri_common(ds, ta);
+    if (order == 3) fill_tri_common(ds, ta);
     PlainObjArgs po;
     std::memset(&po, 0, sizeof po);
     po.singles = ds->singles;
     po.pairs = ds->pairs;
+    po.order = order;
     po.n = n;
     po.N0 = ds->N[0];
     po.N1 = ds->N[1];
     po.score = score;
     po.t64 = ds->tab[score].t64;
-    po.best_key = &ds->ctl->best64;
-    po.cand = ds->cand;
-    po.cand_s64 = ds->cand_s64;
-    po.cand_count = &ds->ctl->cand_count;
-    po.cand_cap = ds->cand_cap;
+    po.nan_flag = (unsigned *)flag;
     const Tri3Tile t = TRI_TILES[plain_tile];
     const int grid_count = (256 / (16 * t.A)) * (256 / (16 * t.B));
-    long long rounds = 0;
+    long long rounds = 0, per_stream[NUM_STREAMS] = {};
     const long long limit = o->plain_rounds > 0 ? o->plain_rounds : -1;
+    // one round: count kernel, then the objective kernel, on stream s
+    auto round = [&](int X, int yb, int zb) -> int {
+        const int s = (int)(rounds % NUM_STREAMS);
+        uint32_t *c = counts + slot * s;
+        if (order == 3) {
+            ta.X = X;
+            ta.yb = yb;
+            ta.zb = zb;
+            ta.counts = c;
+            CHK(dispatch_tri<MODE_COUNTS>(ds, plain_tile, ta, grid_count, ss.st[s]));
+        } else {
+            hipLaunchKernelGGL(k_pairs_counts, dim3(16), dim3(256), 0, ss.st[s], ds->planes[0], ds->W[0],
+                               ds->planes[1], ds->W[1], yb, zb, c);
+        }
+        const long long k = per_stream[s];
+        po.counts = c;
+        po.X = X;
+        po.yb = yb;
+        po.zb = zb;
+        po.blk_out = blk + ((size_t)s * 2 + (k & 1)) * 512;
+        po.blk_in = k ? blk + ((size_t)s * 2 + ((k - 1) & 1)) * 512 : nullptr;
+        po.best = sbest + 2 * s;
+        hipLaunchKernelGGL(k_objective_plain, dim3(256), dim3(256), 0, ss.st[s], po);
+        HIPCHK(hipGetLastError());
+        per_stream[s]++;
+        rounds++;
+        return EPISDET_OK;
+    };
     bool stop = false;
-    for (int yb = 0; yb < nb && !stop; yb++) {
-        for (int X = 0; X < std::min((yb + 1) * BLOCK, n) && !stop; X++) {
-            for (int zb = yb; zb < nb; zb++) {
+    if (order == 3) {
+        for (int yb = 0; yb < nb && !stop; yb++)
+            for (int X = 0; X < std::min((yb + 1) * BLOCK, n) && !stop; X++)
+                for (int zb = yb; zb < nb; zb++) {
+                    if (limit >= 0 && rounds >= limit) {
+                        stop = true;
+                        break;
+                    }
+                    CHK(round(X, yb, zb));
+                }
+    } else {
+        for (int ab = 0; ab < nb && !stop; ab++)
+            for (int bb = ab; bb < nb; bb++) {
                 if (limit >= 0 && rounds >= limit) {
                     stop = true;
                     break;
                 }
-                const int s = (int)(rounds % NUM_STREAMS);
-                uint32_t *c = counts + slot * s;
-                ta.X = X;
-                ta.yb = yb;
-                ta.zb = zb;
-                ta.counts = c;
-                CHK(dispatch_tri<MODE_COUNTS>(ds, plain_tile, ta, grid_count, st[s]));
-                po.counts = c;
-                po.X = X;
-                po.yb = yb;
-                po.zb = zb;
-                hipLaunchKernelGGL(k_triplets_objective_plain, dim3(256), dim3(256), 0, st[s], po);
-                HIPCHK(hipGetLastError());
-                rounds++;
+                CHK(round(0, ab, bb));
             }
-        }
     }
     for (int i = 0; i < NUM_STREAMS; i++) {
-        HIPCHK(hipEventRecord(ev[i], st[i]));
-        HIPCHK(hipStreamWaitEvent(ds->stream, ev[i], 0));
+        if (per_stream[i]) {
+            hipLaunchKernelGGL(k_plain_fold, dim3(1), dim3(64), 0, ss.st[i],
+                               blk + ((size_t)i * 2 + ((per_stream[i] - 1) & 1)) * 512, sbest + 2 * i);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(ss.ev[i], ss.st[i]));
+        HIPCHK(hipStreamWaitEvent(ds->stream, ss.ev[i], 0));
     }
     HIPCHK(hipEventRecord(e1, ds->stream));
     HIPCHK(hipEventSynchronize(e1));
     float kms = 0;
     HIPCHK(hipEventElapsedTime(&kms, e0, e1));
-    res->kernel_ms = kms;
-    res->rounds_run = rounds;
-    for (int i = 0; i < NUM_STREAMS; i++) {
-        (void)hipEventDestroy(ev[i]);
-        (void)hipStreamDestroy(st[i]);
-    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipFree(counts);
+    res->kernel_ms = kms;
+    res->rounds_run = rounds;
 
-    Ctl hc;
-    HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
-    res->candidates = hc.cand_count;
-    if (hc.cand_count > ds->cand_cap) return EPISDET_ERR_CANDIDATES;
-    const int count = (int)hc.cand_count;
-    std::vector<unsigned long long> hp(count);
-    std::vector<double> h64(count);
-    if (count) {
-        HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
-    }
-    double bb = INFINITY;
-    unsigned long long bp = ~0ull;
-    for (int i = 0; i < count; i++) {
-        if (std::isnan(h64[i])) return EPISDET_ERR_INTERNAL;   // inconsistent count tables
-        if (h64[i] < bb || (h64[i] == bb && hp[i] < bp)) {
-            bb = h64[i];
-            bp = hp[i];
+    unsigned long long hb[NUM_STREAMS * 2], hflag = 0;
+    HIPCHK(hipMemcpy(hb, sbest, sizeof hb, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&hflag, flag, sizeof hflag, hipMemcpyDeviceToHost));
+    if (hflag) return EPISDET_ERR_INTERNAL;   // inconsistent count tables
+    unsigned long long bk = ~0ull, bp = ~0ull;
+    for (int i = 0; i < NUM_STREAMS; i++)
+        if (hb[2 * i] < bk || (hb[2 * i] == bk && hb[2 * i + 1] < bp)) {
+            bk = hb[2 * i];
+            bp = hb[2 * i + 1];
         }
-    }
-    res->best_score = score == EPISDET_SCORE_K2 ? bb : -bb;
+    if (bp == ~0ull) return rounds ? EPISDET_ERR_INTERNAL : EPISDET_ERR_INVALID;
+    const double bad = key2d_host(bk);
+    res->best_score = score == EPISDET_SCORE_K2 ? bad : -bad;
     res->best_score_f32 = (float)res->best_score;
     res->packed = bp;
     unpack(order, bp, res->idx);
+    res->candidates = 0;
     res->path = EPISDET_PATH_PLAIN;
     return EPISDET_OK;
 }
 
 long long sets_total(int order, int n) { return (long long)episdet_n_choose_k((unsigned)n, (unsigned)order); }
-
-struct DevList {   // device allocations released together
-    std::vector<void *> ptrs;
-    ~DevList()
-    {
-        for (void *p : ptrs) (void)hipFree(p);
-    }
-    template <typename T>
-    int put(T **dst, const T *src, size_t n)
-    {
-        if (hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return EPISDET_ERR_ALLOC;
-        ptrs.push_back((void *)*dst);
-        if (n && hipMemcpy((void *)*dst, (const void *)src, sizeof(T) * n, hipMemcpyHostToDevice) != hipSuccess)
-            return EPISDET_ERR_HIP;
-        return EPISDET_OK;
-    }
-};
 
 }  // namespace
 
@@ -1548,6 +1694,12 @@ int episdet_search(int order, int score, const unsigned *cases, const unsigned *
     episdet_dataset *ds = nullptr;
     CHK(episdet_dataset_create(cases, controls, numSNPs, numCases, numControls, numCasesWithPadding / 32,
                                numControlsWithPadding / 32, &ds));
+    // The reference prints one line per outer block of its host loop from
+    // inside the search (search-pairs.cu:137-139, search-triplets.cu:162-164).
+    // The search here is one persistent launch, so the lines come first.
+    for (long long start = 0; start < (long long)numSNPsWithPadding; start += BLOCK)
+        std::printf("Outer loop iteration %d out of %d\n", (int)(start / BLOCK), (int)(numSNPsWithPadding / BLOCK));
+    std::fflush(stdout);
     episdet_options o;
     std::memset(&o, 0, sizeof o);
     o.order = order;
@@ -1565,6 +1717,17 @@ int episdet_search(int order, int score, const unsigned *cases, const unsigned *
 }
 
 // ------------------------------------------------------ batched search --
+
+// Candidate slots of a batched job: the requested capacity (default 1M),
+// never fewer than one work unit's sets of its MFMA tile.
+static unsigned batch_cap(const episdet_options &o, int order, int tile)
+{
+    const unsigned long long unit = order == 3 ? (unsigned long long)mfma_xstep(tile) * mfma_ty(tile) * mfma_tz(tile)
+                                               : (tile == 0 ? (unsigned long long)P2_TA * P2_TB
+                                                            : (unsigned long long)P3_TA * P3_TB);
+    const unsigned long long want = o.candidate_cap > 0 ? (unsigned long long)o.candidate_cap : DEFAULT_CAP;
+    return (unsigned)std::max(want, unit);
+}
 
 int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, int njobs, episdet_result *results,
                       double *batch_ms)
@@ -1634,6 +1797,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             continue;
         }
         seen.push_back(ds);
+        CHK(ensure_cand(ds, batch_cap(o, o.order, ktile[j])));
         if (o.order == 3) tri[ktile[j]].push_back(j);
         else pairs[ktile[j]].push_back(j);
     }
@@ -1656,6 +1820,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             CHK(dl.put(&d_groups, groups.data(), groups.size()));
             TriArgs &a = ta[tl][k];
             fill_tri_common(ds, a);
+            a.co.cand_cap = batch_cap(opts[tri[tl][k]], 3, tl);
             a.sc = make_ctx(ds, opts[tri[tl][k]].score, 3);
             a.gstart = d_gstart;
             a.groups = d_groups;
@@ -1672,6 +1837,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         for (size_t k = 0; k < pairs[tl].size(); k++) {
             episdet_dataset *ds = dss[pairs[tl][k]];
             fill_pairs(ds, pa[tl][k], true, tl);
+            pa[tl][k].co.cand_cap = batch_cap(opts[pairs[tl][k]], 2, tl);
             pa[tl][k].sc = make_ctx(ds, opts[pairs[tl][k]].score, 2);
             pstart[tl].push_back(pstart[tl].back() + pa[tl][k].nunits);
         }
@@ -1701,7 +1867,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         r.cand = ds->cand;
         r.out_s64 = ds->cand_s64;
         rj[k].count = &ds->ctl->cand_count;
-        rj[k].cap = ds->cand_cap;
+        rj[k].cap = batch_cap(opts[batched[k]], r.order, ktile[batched[k]]);
         qp[k] = &ds->ctl->queue;
         bp[k] = &ds->ctl->best32;
         cp[k] = &ds->ctl->cand_count;
@@ -1804,13 +1970,16 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         episdet_dataset *ds = dss[j];
         Ctl hc;
         HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
-        if (hc.cand_count == 0 || hc.cand_count > ds->cand_cap) {
+        if (hc.cand_count == 0 || hc.cand_over || hc.cand_count > batch_cap(opts[j], opts[j].order, ktile[j])) {
             solo.push_back(j);     // overflow: exact rerun on its own
             continue;
         }
         episdet_result &r = results[j];
         CHK(collect_best(ds, opts[j].order, opts[j].score, (int)hc.cand_count, &r));
+        // the graph's device time; its re-scoring kernel is part of it
+        // (rescore_ms stays 0 for batched jobs)
         r.kernel_ms = gms;
+        r.tile = ktile[j];
         r.margin = make_ctx(ds, opts[j].score, opts[j].order).margin;
         r.path = EPISDET_PATH_FUSED_MFMA;
         r.rounds = episdet_reference_rounds(opts[j].order, ds->n);

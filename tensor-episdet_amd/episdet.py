@@ -381,10 +381,11 @@ class Dataset:
                 "sample_sets": e.sample_sets} for e in arr[:min(cnt.value, cap)]]
         return out, chosen.value
 
-    def topk(self, order, k, score=K2, path=PATH_AUTO, tile=0, rebuild=False):
+    def topk(self, order, k, score=K2, path=PATH_AUTO, tile=0, rebuild=False, candidate_cap=0):
         """The k best sets by FP64 score (best first): (packed list, scores
         list, result dict)."""
-        o = Options(order=order, score=score, path=path, tile=tile, rebuild=1 if rebuild else 0)
+        o = Options(order=order, score=score, path=path, tile=tile, rebuild=1 if rebuild else 0,
+                    candidate_cap=candidate_cap)
         packed = (C.c_ulonglong * k)()
         scores = (C.c_double * k)()
         cnt = C.c_int()
