@@ -278,6 +278,14 @@ double orc_score_mi(const uint32_t *cas, const uint32_t *ctl, int ncells, int Nc
 
 uint64_t orc_table_hash(uint64_t rank, const uint32_t *cas, const uint32_t *ctl, int ncells);
 
+static inline uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
 typedef struct {
     double score;       /* FP64 score of the winner                          */
     uint64_t packed;    /* X | Y<<21 | Z<<42, or A | B<<32                   */
@@ -386,7 +394,8 @@ typedef struct {
     int n, N[2], W64[2];
     uint64_t *q[2];      /* packed planes per class                 */
     uint32_t *single;    /* [class][snp][3]                         */
-    uint32_t *pair;      /* [class][a][b][4] AND counts, a < b < pn */
+    uint32_t *pair;      /* [class][b][a][4] AND counts, a < b < pn: contiguous in the
+                            smaller index, which the search's innermost loop walks */
     int pn;
 } ctx_t;
 
@@ -400,7 +409,7 @@ static void ctx_pairs(ctx_t *cx, int pn)
         #pragma omp parallel for schedule(dynamic, 4)
         for (int a = 0; a < pn; a++)
             for (int b = a + 1; b < pn; b++) {
-                uint32_t *o = cx->pair + (((size_t)c * pn + a) * pn + b) * 4;
+                uint32_t *o = cx->pair + (((size_t)c * pn + b) * pn + a) * 4;
                 for (int i = 0; i < 2; i++)
                     for (int j = 0; j < 2; j++)
                         o[i * 2 + j] = popc_and(q + ((size_t)a * 2 + i) * W64, q + ((size_t)b * 2 + j) * W64, W64);
@@ -410,7 +419,7 @@ static void ctx_pairs(ctx_t *cx, int pn)
 
 static void ctx_pair9(const ctx_t *cx, int c, int a, int b, uint32_t t[9])
 {
-    pair9_from4(cx->pair + (((size_t)c * cx->pn + a) * cx->pn + b) * 4, cx->single + ((size_t)c * cx->n + a) * 3,
+    pair9_from4(cx->pair + (((size_t)c * cx->pn + b) * cx->pn + a) * 4, cx->single + ((size_t)c * cx->n + a) * 3,
                 cx->single + ((size_t)c * cx->n + b) * 3, t);
 }
 
@@ -420,6 +429,56 @@ static double score_tables(int order, int score, const uint32_t *cas, const uint
     int nc = order == 3 ? 27 : 9;
     if (score == ORC_K2) return orc_score_k2(cas, ctl, nc, tab);
     return orc_score_mi(cas, ctl, nc, cx->N[0], cx->N[1], tab);
+}
+
+/* Eight sets at a time: the per-set operation sequence is exactly that of
+ * orc_score_k2 / orc_score_mi / orc_table_hash (so every result is the same
+ * to the bit), with the eight independent dependency chains interleaved. */
+#define ILV 8
+
+static void score8_k2(const uint32_t cas[ILV][27], const uint32_t ctl[ILV][27], int ncells, const double *T,
+                      double out[ILV])
+{
+    double s[ILV];
+    for (int j = 0; j < ILV; j++) s[j] = 0.0;
+    for (int c = 0; c < ncells; c++)
+        for (int j = 0; j < ILV; j++) {
+            double t = T[ctl[j][c]] + T[cas[j][c]];
+            t = t - T[ctl[j][c] + cas[j][c] + 1];
+            s[j] = s[j] + t;
+        }
+    for (int j = 0; j < ILV; j++) out[j] = fabs(s[j]);
+}
+
+static void score8_mi(const uint32_t cas[ILV][27], const uint32_t ctl[ILV][27], int ncells, int Nc, int Nt,
+                      const double *L, double out[ILV])
+{
+    double invN = 1.0 / (double)(Nc + Nt);
+    double hY = -((double)Nc * invN) * L[Nc];
+    hY = hY - ((double)Nt * invN) * L[Nt];
+    double hG[ILV], hGY[ILV];
+    for (int j = 0; j < ILV; j++) hG[j] = hGY[j] = 0.0;
+    for (int c = 0; c < ncells; c++)
+        for (int j = 0; j < ILV; j++) {
+            double pc = (double)cas[j][c] * invN;
+            double pt = (double)ctl[j][c] * invN;
+            hGY[j] = hGY[j] - pc * L[cas[j][c]];
+            hGY[j] = hGY[j] - pt * L[ctl[j][c]];
+            hG[j] = hG[j] - (pc + pt) * L[cas[j][c] + ctl[j][c]];
+        }
+    for (int j = 0; j < ILV; j++) out[j] = (hG[j] + hY) - hGY[j];
+}
+
+static void hash8(const uint64_t rank[ILV], const uint32_t cas[ILV][27], const uint32_t ctl[ILV][27], int ncells,
+                  uint64_t out[ILV])
+{
+    uint64_t h[ILV];
+    for (int j = 0; j < ILV; j++) h[j] = (rank[j] + 1) * 0x9E3779B97F4A7C15ull;
+    for (int c = 0; c < ncells; c++)
+        for (int j = 0; j < ILV; j++) h[j] = mix64(h[j] ^ cas[j][c]);
+    for (int c = 0; c < ncells; c++)
+        for (int j = 0; j < ILV; j++) h[j] = mix64(h[j] ^ ctl[j][c]);
+    for (int j = 0; j < ILV; j++) out[j] = h[j];
 }
 
 /* Exhaustive search.  cases/controls: [n][2][Wc] / [n][2][Wt] u32 planes.
@@ -474,6 +533,12 @@ int orc_search_ex(const uint32_t *cases, const uint32_t *controls, int n, int Nc
         int W64c = cx.W64[0], W64t = cx.W64[1];
         size_t qmax = (size_t)(W64c > W64t ? W64c : W64t);
         uint64_t *Q = (uint64_t *)malloc(sizeof(uint64_t) * 4 * qmax);
+        /* X rows per block of the 3-way loop: both planes of both classes of
+         * XB SNPs in about 1 MiB, a multiple of ILV between ILV and 64 */
+        int XB = (int)((1u << 20) / ((size_t)(W64c + W64t) * 2 * sizeof(uint64_t)));
+        XB = XB / ILV * ILV;
+        if (XB < ILV) XB = ILV;
+        if (XB > 64) XB = 64;
         #pragma omp for schedule(dynamic, 1)
         for (int hi = last_hi - 1; hi >= last_lo; hi--) {
             if (order == 2) {
@@ -500,14 +565,19 @@ int orc_search_ex(const uint32_t *cases, const uint32_t *controls, int n, int Nc
             } else {
                 int Z = hi;
                 const uint64_t rz3 = (uint64_t)Z * (Z - 1) * (Z - 2) / 6;   /* colex rank base C(Z,3) */
-                for (int Y = 1; Y < Z; Y++) {
-                    /* Q[j*2+k] = Y_j & Z_k, per class, reused over every X < Y. */
-                    uint32_t n8[2][64][8];
-                    uint32_t yz[2][9];
-                    for (int c = 0; c < 2; c++) ctx_pair9(&cx, c, Y, Z, yz[c]);
-                    for (int X0 = 0; X0 < Y; X0 += 64) {
-                        int X1 = X0 + 64 < Y ? X0 + 64 : Y;
+                /* Blocked over X: a block of XB X rows (both classes, about
+                 * 1 MiB) stays in the core's cache while every Y above it is
+                 * contracted against it; X rows are then read from memory
+                 * once per (block, Z) instead of once per (Y, Z). */
+                for (int X0 = 0; X0 < Z - 1; X0 += XB) {
+                    const int XE = X0 + XB < Z - 1 ? X0 + XB : Z - 1;
+                    for (int Y = X0 + 1; Y < Z; Y++) {
+                        const int X1 = XE < Y ? XE : Y;      /* X in [X0, X1), X < Y */
+                        uint32_t n8[2][64][8];
+                        uint32_t yz[2][9];
+                        for (int c = 0; c < 2; c++) ctx_pair9(&cx, c, Y, Z, yz[c]);
                         for (int c = 0; c < 2; c++) {
+                            /* Q[j*2+k] = Y_j & Z_k of this class */
                             int W64 = cx.W64[c];
                             const uint64_t *q = cx.q[c];
                             for (int j = 0; j < 2; j++)
@@ -521,20 +591,35 @@ int orc_search_ex(const uint32_t *cases, const uint32_t *controls, int n, int Nc
                                 count8(q + ((size_t)X * 2) * W64, q + ((size_t)X * 2 + 1) * W64, Q, qmax, W64,
                                        n8[c][X - X0]);
                         }
-                        for (int X = X0; X < X1; X++) {
-                            uint32_t cas[27], ctl[27], xy[9], xz[9];
-                            ctx_pair9(&cx, 0, X, Y, xy);
-                            ctx_pair9(&cx, 0, X, Z, xz);
-                            triplet27_from8(n8[0][X - X0], xy, xz, yz[0], cas);
-                            ctx_pair9(&cx, 1, X, Y, xy);
-                            ctx_pair9(&cx, 1, X, Z, xz);
-                            triplet27_from8(n8[1][X - X0], xy, xz, yz[1], ctl);
-                            double s = score_tables(3, score, cas, ctl, &cx, tab);
-                            uint64_t p = (uint64_t)X | ((uint64_t)Y << 21) | ((uint64_t)Z << 42);
-                            if (want_hash)
-                                th += orc_table_hash(rz3 + (uint64_t)Y * (Y - 1) / 2 + (uint64_t)X, cas, ctl, 27);
-                            if (better(score, s, p, tb, tp)) { tb = s; tp = p; }
-                            ts++;
+                        for (int Xb = X0; Xb < X1; Xb += ILV) {
+                            /* ILV sets at once (the last group repeats its final X) */
+                            const int m = X1 - Xb < ILV ? X1 - Xb : ILV;
+                            uint32_t cas[ILV][27], ctl[ILV][27];
+                            uint64_t rk[ILV], hv[ILV];
+                            double sv[ILV];
+                            for (int j = 0; j < ILV; j++) {
+                                const int X = Xb + (j < m ? j : m - 1);
+                                uint32_t xy[9], xz[9];
+                                ctx_pair9(&cx, 0, X, Y, xy);
+                                ctx_pair9(&cx, 0, X, Z, xz);
+                                triplet27_from8(n8[0][X - X0], xy, xz, yz[0], cas[j]);
+                                ctx_pair9(&cx, 1, X, Y, xy);
+                                ctx_pair9(&cx, 1, X, Z, xz);
+                                triplet27_from8(n8[1][X - X0], xy, xz, yz[1], ctl[j]);
+                                rk[j] = rz3 + (uint64_t)Y * (Y - 1) / 2 + (uint64_t)X;
+                            }
+                            if (score == ORC_K2) score8_k2(cas, ctl, 27, tab, sv);
+                            else score8_mi(cas, ctl, 27, cx.N[0], cx.N[1], tab, sv);
+                            if (want_hash) {
+                                hash8(rk, cas, ctl, 27, hv);
+                                for (int j = 0; j < m; j++) th += hv[j];
+                            }
+                            for (int j = 0; j < m; j++) {
+                                const int X = Xb + j;
+                                uint64_t p = (uint64_t)X | ((uint64_t)Y << 21) | ((uint64_t)Z << 42);
+                                if (better(score, sv[j], p, tb, tp)) { tb = sv[j]; tp = p; }
+                            }
+                            ts += m;
                         }
                     }
                 }
@@ -578,13 +663,6 @@ int orc_search(const uint32_t *cases, const uint32_t *controls, int n, int Nc, i
 
 /* ---------------------------------------------------- full-table checking */
 
-static inline uint64_t mix64(uint64_t z)
-{
-    z += 0x9E3779B97F4A7C15ull;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
 
 /* Order-independent checksum of every set's table:
  *   h = (rank+1) * 0x9E3779B97F4A7C15; h = mix64(h ^ cell) over the cases
