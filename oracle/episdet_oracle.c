@@ -327,68 +327,85 @@ static inline uint32_t popc_and(const uint64_t *a, const uint64_t *b, int W64)
     return (uint32_t)c;
 }
 
-/* out[i*4+jk] = popcount(x_i & Q_jk) for i in {0,1}, jk in 0..3, one pass
- * over the words.  Q holds four rows of qs words each. */
-static void count8_scalar(const uint64_t *x0, const uint64_t *x1, const uint64_t *Q, size_t qs, int W64,
-                          uint32_t *out)
+/* Two X rows against one Q at once: for X in {a, b} (plane 0 at x, plane 1
+ * at x + W64), out[i*4+jk] = popcount(X_i & Q_jk), i in {0,1}, jk in 0..3,
+ * in one pass over the words.  Q holds four rows of qs words each.  Each Q
+ * word is loaded once for both rows. */
+static void count16_scalar(const uint64_t *xa, const uint64_t *xb, const uint64_t *Q, size_t qs, int W64,
+                           uint32_t *oa, uint32_t *ob)
 {
-    uint64_t c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint64_t c[16] = {0};
     for (int w = 0; w < W64; w++) {
-        uint64_t a = x0[w], b = x1[w];
+        const uint64_t a0 = xa[w], a1 = xa[W64 + w], b0 = xb[w], b1 = xb[W64 + w];
         for (int jk = 0; jk < 4; jk++) {
-            uint64_t q = Q[jk * qs + w];
-            c[jk] += (uint64_t)__builtin_popcountll(a & q);
-            c[4 + jk] += (uint64_t)__builtin_popcountll(b & q);
+            const uint64_t q = Q[jk * qs + w];
+            c[jk] += (uint64_t)__builtin_popcountll(a0 & q);
+            c[4 + jk] += (uint64_t)__builtin_popcountll(a1 & q);
+            c[8 + jk] += (uint64_t)__builtin_popcountll(b0 & q);
+            c[12 + jk] += (uint64_t)__builtin_popcountll(b1 & q);
         }
     }
-    for (int i = 0; i < 8; i++) out[i] = (uint32_t)c[i];
+    for (int i = 0; i < 8; i++) {
+        oa[i] = (uint32_t)c[i];
+        ob[i] = (uint32_t)c[8 + i];
+    }
 }
 
 #if defined(__x86_64__)
 #include <immintrin.h>
 __attribute__((target("avx512f,avx512vpopcntdq")))
-static void count8_avx512(const uint64_t *x0, const uint64_t *x1, const uint64_t *Q, size_t qs, int W64,
-                          uint32_t *out)
+static void count16_avx512(const uint64_t *xa, const uint64_t *xb, const uint64_t *Q, size_t qs, int W64,
+                           uint32_t *oa, uint32_t *ob)
 {
-    __m512i c[8];
-    for (int i = 0; i < 8; i++) c[i] = _mm512_setzero_si512();
+    __m512i c[16];
+    for (int i = 0; i < 16; i++) c[i] = _mm512_setzero_si512();
     int w = 0;
     for (; w + 8 <= W64; w += 8) {
-        __m512i a = _mm512_loadu_si512((const void *)(x0 + w));
-        __m512i b = _mm512_loadu_si512((const void *)(x1 + w));
+        const __m512i a0 = _mm512_loadu_si512((const void *)(xa + w));
+        const __m512i a1 = _mm512_loadu_si512((const void *)(xa + W64 + w));
+        const __m512i b0 = _mm512_loadu_si512((const void *)(xb + w));
+        const __m512i b1 = _mm512_loadu_si512((const void *)(xb + W64 + w));
         for (int jk = 0; jk < 4; jk++) {
-            __m512i q = _mm512_loadu_si512((const void *)(Q + jk * qs + w));
-            c[jk] = _mm512_add_epi64(c[jk], _mm512_popcnt_epi64(_mm512_and_si512(a, q)));
-            c[4 + jk] = _mm512_add_epi64(c[4 + jk], _mm512_popcnt_epi64(_mm512_and_si512(b, q)));
+            const __m512i q = _mm512_loadu_si512((const void *)(Q + jk * qs + w));
+            c[jk] = _mm512_add_epi64(c[jk], _mm512_popcnt_epi64(_mm512_and_si512(a0, q)));
+            c[4 + jk] = _mm512_add_epi64(c[4 + jk], _mm512_popcnt_epi64(_mm512_and_si512(a1, q)));
+            c[8 + jk] = _mm512_add_epi64(c[8 + jk], _mm512_popcnt_epi64(_mm512_and_si512(b0, q)));
+            c[12 + jk] = _mm512_add_epi64(c[12 + jk], _mm512_popcnt_epi64(_mm512_and_si512(b1, q)));
         }
     }
-    uint64_t t[8];
-    for (int i = 0; i < 8; i++) t[i] = (uint64_t)_mm512_reduce_add_epi64(c[i]);
+    uint64_t t[16];
+    for (int i = 0; i < 16; i++) t[i] = (uint64_t)_mm512_reduce_add_epi64(c[i]);
     for (; w < W64; w++) {
-        uint64_t a = x0[w], b = x1[w];
+        const uint64_t a0 = xa[w], a1 = xa[W64 + w], b0 = xb[w], b1 = xb[W64 + w];
         for (int jk = 0; jk < 4; jk++) {
-            uint64_t q = Q[jk * qs + w];
-            t[jk] += (uint64_t)__builtin_popcountll(a & q);
-            t[4 + jk] += (uint64_t)__builtin_popcountll(b & q);
+            const uint64_t q = Q[jk * qs + w];
+            t[jk] += (uint64_t)__builtin_popcountll(a0 & q);
+            t[4 + jk] += (uint64_t)__builtin_popcountll(a1 & q);
+            t[8 + jk] += (uint64_t)__builtin_popcountll(b0 & q);
+            t[12 + jk] += (uint64_t)__builtin_popcountll(b1 & q);
         }
     }
-    for (int i = 0; i < 8; i++) out[i] = (uint32_t)t[i];
+    for (int i = 0; i < 8; i++) {
+        oa[i] = (uint32_t)t[i];
+        ob[i] = (uint32_t)t[8 + i];
+    }
 }
 #endif
 
-typedef void (*count8_fn)(const uint64_t *, const uint64_t *, const uint64_t *, size_t, int, uint32_t *);
+typedef void (*count16_fn)(const uint64_t *, const uint64_t *, const uint64_t *, size_t, int, uint32_t *,
+                           uint32_t *);
 
-static count8_fn pick_count8(void)
+static count16_fn pick_count16(void)
 {
 #if defined(__x86_64__)
     __builtin_cpu_init();
-    if (__builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512vpopcntdq")) return count8_avx512;
+    if (__builtin_cpu_supports("avx512f") && __builtin_cpu_supports("avx512vpopcntdq")) return count16_avx512;
 #endif
-    return count8_scalar;
+    return count16_scalar;
 }
 
 /* 1 when the AVX-512 VPOPCNTDQ counting loop is in use on this host. */
-int orc_uses_avx512(void) { return pick_count8() != count8_scalar; }
+int orc_uses_avx512(void) { return pick_count16() != count16_scalar; }
 
 typedef struct {
     int n, N[2], W64[2];
@@ -523,7 +540,7 @@ int orc_search_ex(const uint32_t *cases, const uint32_t *controls, int n, int Nc
     uint64_t hsum = 0;
     const int want_hash = checksum != NULL;
 
-    count8_fn count8 = pick_count8();
+    count16_fn count16 = pick_count16();
     #pragma omp parallel
     {
         double tb = score == ORC_K2 ? INFINITY : -INFINITY;
@@ -587,9 +604,12 @@ int orc_search_ex(const uint32_t *cases, const uint32_t *controls, int n, int Nc
                                     uint64_t *d = Q + (size_t)(j * 2 + k) * qmax;
                                     for (int w = 0; w < W64; w++) d[w] = yj[w] & zk[w];
                                 }
-                            for (int X = X0; X < X1; X++)
-                                count8(q + ((size_t)X * 2) * W64, q + ((size_t)X * 2 + 1) * W64, Q, qmax, W64,
-                                       n8[c][X - X0]);
+                            for (int X = X0; X < X1; X += 2) {
+                                /* an odd last row is paired with itself */
+                                const int Xb = X + 1 < X1 ? X + 1 : X;
+                                count16(q + (size_t)X * 2 * W64, q + (size_t)Xb * 2 * W64, Q, qmax, W64,
+                                        n8[c][X - X0], n8[c][Xb - X0]);
+                            }
                         }
                         for (int Xb = X0; Xb < X1; Xb += ILV) {
                             /* ILV sets at once (the last group repeats its final X) */

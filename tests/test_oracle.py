@@ -194,6 +194,25 @@ def test_search_slices_cover_full_search():
         assert (best["packed"], best["score"]) == (full["packed"], full["score"])
 
 
+@pytest.mark.parametrize("n,N,W", [(7, 64, 2), (75, 700, 32), (150, 96, 3)])
+def test_search_checksum_equals_per_set_tables(n, N, W):
+    """The search's table checksum (X rows blocked and counted two at a time,
+    an odd last row paired with itself) equals the per-set bit-plane count of
+    every table (orc_all_tables), for both orders; 150 SNPs on 3 words span
+    several X blocks of 64."""
+    rng = np.random.default_rng(n + N)
+    _, pc = rand_planes(rng, n, N, W)
+    _, pt = rand_planes(rng, n, N + 5 if N + 5 <= 32 * W else N, W)
+    Nt = N + 5 if N + 5 <= 32 * W else N
+    for order in (2, 3):
+        _, ref = O.all_tables(pc, pt, N, Nt, order, want_tables=False)
+        r = O.search(pc, pt, N, Nt, order, O.K2, checksum=True)
+        assert r["checksum"] == ref
+        parts = [O.search(pc, pt, N, Nt, order, O.MI, checksum=True, last=(lo, min(lo + 11, n)))
+                 for lo in range(order - 1, n, 11)]
+        assert sum(p["checksum"] for p in parts) % 2**64 == ref
+
+
 def test_oracle_recovers_planted_triplet():
     syn = episdet.generate(48, 4000, 4000, seed=7, planted_sets=1, planted_order=3)
     r = O.search(syn.cases, syn.controls, 4000, 4000, 3, O.K2)
