@@ -11,14 +11,23 @@ re-scoring of candidates and the final reduction.
 value      = C(n,3) * (cases + controls) / seconds per step  (the reference's
              metric, tensor-episdet.cu:198, unpadded samples)
 roofline   = the fused kernel alone, from HIP events on its stream:
-             achieved = 0.5 lane-ops per set-sample (VALU path: one v_and_b32
-             and one v_bcnt_u32_b32 per cell per 32 samples, 8 cells) x
-             set-samples / kernel seconds, against 256 CU x 64 lanes x 2.4 GHz
+             FP4-MFMA path (the default below 2^24 samples per class):
+             achieved = 16 flop per 3-way set-sample (8 multiply-adds, one
+             per AND-count cell; 8 flop for 2-way) x set-samples / kernel
+             seconds, against MI355X's dense FP4 peak of 10.1 PFLOP/s;
+             VALU path: 0.5 lane-ops per 3-way set-sample (one v_and_b32 and
+             one v_bcnt_u32_b32 per cell per 32 samples, 8 cells) against
+             256 CU x 64 lanes x 2.4 GHz
+null_data  = the same search on the same shape with no planted set (the
+             score-bound skip depends on how strong the best set is), one
+             step, reported beside the headline
 plain      = the reference-shaped round loop (count kernel -> int32 tables in
              HBM -> FP64 objective kernel), timed on its first R rounds and
              scaled by the round count (all rounds have identical work)
 cpu        = the OpenMP AND+popcount oracle on a contiguous slice of the same
-             search (sets whose largest index lies in a stated range)
+             search (sets whose largest index lies in a stated range), on
+             every host CPU this process may use, at most the OMP_NUM_THREADS
+             share (16 per GPU on the MI355X boxes)
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1|2|3|5] [--path mfma|valu]
 """
@@ -55,6 +64,16 @@ PRESETS = {
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    """CPU threads for the oracle baseline: every CPU this process may run on,
+    capped by the OMP_NUM_THREADS share when one is set; and the host's
+    counts, reported beside it."""
+    aff = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(aff, share) if share > 0 else aff
+    return threads, {"host_cpus": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": share or None}
 
 
 def measured_traffic(n, Nc, Nt, order, score, path, tile):
@@ -138,7 +157,7 @@ def bench_batch(args, episdet):
         # the smallest jobs, whole, one after another, until about --cpu-seconds
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        threads = min(os.cpu_count() or 1, 16)
+        threads, hostinfo = host_threads()
         order_by_size = sorted(range(len(spec)), key=lambda j: episdet.n_choose_k(spec[j][2], spec[j][0]) *
                                (spec[j][3] + spec[j][4]))
         done, ss, t0 = [], 0, time.perf_counter()
@@ -150,7 +169,7 @@ def bench_batch(args, episdet):
             if time.perf_counter() - t0 > args.cpu_seconds:
                 break
         dt = time.perf_counter() - t0
-        cpu = {"value": ss / dt, "unit": "set-samples/s", "cores": threads, "kind": "port",
+        cpu = {"value": ss / dt, "unit": "set-samples/s", "cores": threads, "kind": "port", **hostinfo,
                "sample": f"oracle (OpenMP AND+popcount, FP64 K2/MI): the {len(done)} smallest of the 64 jobs, "
                          f"whole, {dt:.1f} s"}
         log(f"cpu: {len(done)} jobs in {dt:.1f} s on {threads} threads")
@@ -196,6 +215,7 @@ def main():
     ap.add_argument("--plain-rounds", type=int, default=2048)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-null", action="store_true", help="skip the no-planted-set timing")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -268,11 +288,28 @@ def main():
                  "seconds_est": est, "dtype": "f64 scoring", "speedup_fused": (sets * N / est) and value / (sets * N / est)}
         log(f"plain: {rr['rounds_run']} rounds in {rr['kernel_ms']:.1f} ms -> {est:.1f} s estimated")
 
+    null = None
+    if not args.no_null and pre["planted"] and order == 3:
+        # the same shape and generator seed with nothing planted: as many
+        # warmups as the headline, then one timed step
+        ds.close()
+        nsyn = episdet.generate(n, Nc, Nt, seed=args.seed, planted_sets=0, maf=pre["maf"], maf_dist=pre["maf_dist"])
+        ds = episdet.Dataset.from_synthetic(nsyn)
+        for i in range(args.warmup):
+            step()
+        t0 = time.perf_counter()
+        nr = step()
+        ndt = time.perf_counter() - t0
+        null = {"value": sets * N / ndt, "ms_per_step": ndt * 1e3, "kernel_ms": nr["kernel_ms"],
+                "candidates": nr["candidates"], "best": list(nr["idx"][:order]), "best_score": nr["best_score"],
+                "data": "same shape and generator seed, no planted set"}
+        log(f"null data: {ndt:.3f} s (kernel {nr['kernel_ms']:.1f} ms) best {nr['idx'][:order]}")
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
-        threads = min(os.cpu_count() or 1, 16)
+        threads, hostinfo = host_threads()
         # Calibrate a slice of the search (the sets whose largest index is in
         # [z0, z0+zn)) to about --cpu-seconds.  A set's cost is the same
         # everywhere, and C(Z, order-1) sets have largest index Z, so two
@@ -298,6 +335,7 @@ def main():
                 reps += 1
             dt = time.perf_counter() - t0
             cpu = {"value": reps * c["sets"] * N / dt, "unit": "set-samples/s", "cores": threads, "kind": "port",
+                   **hostinfo,
                    "sample": f"oracle (OpenMP AND+popcount, FP64 {SCORE_NAME[score]}): the whole search "
                              f"({c['sets']} sets), {reps} times, {dt:.1f} s"}
             log(f"cpu: whole search x{reps} in {dt:.1f} s on {threads} threads")
@@ -321,7 +359,7 @@ def main():
             c = oracle.search(syn.cases, syn.controls, Nc, Nt, order, score, threads=threads, last=(z0, z0 + zn))
             dt = time.perf_counter() - t0
             cpu = {"value": c["sets"] * N / dt, "unit": "set-samples/s", "cores": threads, "kind": "port",
-                   "sample": f"oracle (OpenMP AND+popcount{', AVX-512 VPOPCNTDQ' if oracle.uses_avx512() else ''}, "
+                   **hostinfo, "sample": f"oracle (OpenMP AND+popcount{', AVX-512 VPOPCNTDQ' if oracle.uses_avx512() else ''}, "
                              f"FP64 {SCORE_NAME[score]}) on the {c['sets']} sets whose largest index is in "
                              f"[{z0},{z0 + zn}), {dt:.1f} s"}
             log(f"cpu: {c['sets']} sets in {dt:.1f} s on {threads} threads")
@@ -351,6 +389,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "plain_path": plain,
+        "null_data": null,
         "best": {"idx": list(r["idx"][:order]), "score": r["best_score"], "planted_recovered": planted_ok,
                  "candidates": r["candidates"], "fp32_max_rel_err": r["fp32_max_rel_err"]},
     }

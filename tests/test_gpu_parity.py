@@ -331,47 +331,6 @@ def test_full_sample_axis_checksums_agree():
         assert ds.checksum(3, path=episdet.PATH_FUSED_VALU) == c4
 
 
-# Checksum of all 11,444,858,880 (set, table) pairs of the benchmark workload
-# (generate(4096, 131072, 131072, seed=1, planted_sets=1)): the value MFMA
-# tiles 4, 0 and 5 and the VALU path all returned in one run (VALU: 68 s).
-BENCH_WORKLOAD_CHECKSUM = 0x4F1B9FCBA564E85F
-
-
-def test_benchmark_workload_checksum():
-    """The benchmark workload at full size, 4096 SNPs x 262144 samples: the
-    order-independent checksum of every triplet's 54-cell table from the
-    default kernel (tile 4) and from tile 0 (64 x 64 units) equals the value
-    all four kernels agreed on; the single-best and top-k searches agree."""
-    syn = episdet.generate(4096, 131072, 131072, seed=1, planted_sets=1, planted_order=3)
-    with episdet.Dataset.from_synthetic(syn) as ds:
-        for tile in (4, 0):
-            say(f"benchmark workload checksum: MFMA tile {tile} (about 20 s)")
-            assert ds.checksum(3, path=episdet.PATH_FUSED_MFMA, tile=tile) == BENCH_WORKLOAD_CHECKSUM
-        # single best (seeded, skipping against the best) and top-k (skipping
-        # against the k-th best) agree on the winner; it is the planted set
-        say("benchmark workload: single best and top-16")
-        r = ds.run(3, episdet.K2)
-        p, sc, _ = ds.topk(3, 16, score=episdet.K2)
-        assert r["seeded"] == 1 and r["packed"] == p[0] and r["best_score"] == sc[0]
-        assert tuple(r["idx"]) == syn.planted[0]
-        assert sc == sorted(sc) and len(set(p)) == 16
-
-
-def test_pair_workload_checksums_agree():
-    """The 2-way benchmark workload at full size (16384 SNPs, 26214 cases +
-    39322 controls, MAF ~ Beta(0.5, 2)): the checksum of all 134,209,536 pair
-    tables is the same from both MFMA pair kernels and the VALU path."""
-    syn = episdet.generate(16384, 26214, 39322, seed=1, planted_sets=1, planted_order=2,
-                           maf=(0.01, 0.5), maf_dist=episdet.MAF_BETA)
-    with episdet.Dataset.from_synthetic(syn) as ds:
-        say("2-way workload checksum: MFMA tile 1")
-        c1 = ds.checksum(2, path=episdet.PATH_FUSED_MFMA, tile=1)
-        say("2-way workload checksum: MFMA tile 0")
-        assert ds.checksum(2, path=episdet.PATH_FUSED_MFMA, tile=0) == c1
-        say("2-way workload checksum: VALU")
-        assert ds.checksum(2, path=episdet.PATH_FUSED_VALU) == c1
-
-
 def test_mi_pairs_uneven_classes_planted():
     """Config-3 style: 40/60 classes with padding that is not a 1024-bit
     multiple of the valid bits, MAF ~ Beta(0.5, 2), planted pair."""
