@@ -17,7 +17,8 @@ struct ScoreCtx {
     float k0;           // K2: ln(N)/2 - 1 + ln(2 pi)/2 (see k2_term)
     float c;            // K2: ln N - 1
     float invN;         // 1/N
-    float hY;           // MI: H(Y) in bits
+    float mi_q[2];      // MI: Nc/N and 1 - Nc/N, the expected share of each class
+    float mi_scale;     // MI: 1 / (N ln 2)
     float margin;       // candidate margin, 2 x the FP32 error bound
     // Lower bounds of the badness (bound_exceeds): lb_max - lb_scale * S, and
     // for K2 that plus lb_lnw * sum_c log2 max(n_c, 1) + lb_off, with S the
@@ -137,15 +138,6 @@ __device__ __forceinline__ float k2_term(uint32_t m, float invN, float k0, float
     return fmaf(fmaf(x, LN2_F, -0.5f * LN2_F), l2, k0);
 }
 
-// MI term p log2 p, p = m/N (0 for m = 0), from v_log_f32; its error is
-// measured like k2_term's.
-__device__ __forceinline__ float plogp(uint32_t m, float invN)
-{
-#pragma clang fp contract(off)
-    const float p = (float)m * invN;
-    return m ? p * __builtin_amdgcn_logf(p) : 0.f;
-}
-
 // K2 in FP32.  Each cell contributes f = ln ctl! + ln cas! - ln (ctl+cas+1)!
 // (scoring.hpp:5-7), always <= 0, so K2 = -sum f.  With T'(m) = ln m! - c*m
 // and cas + ctl - (cas+ctl+1) = -1, every cell reads f + c, and
@@ -162,18 +154,61 @@ __device__ __forceinline__ float k2_f32(const uint32_t *cas, const uint32_t *ctl
     return total - s;
 }
 
-// MI in FP32: H(G) + H(Y) - H(G,Y) in bits (scoring.hpp:9-16), with
-// compensated sums.
-__device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, float invN, float hY)
+// MI in FP32 from non-negative terms.  With x a cell's count in one class
+// and y = q n_c the count expected without association (q = Nc/N for the
+// cases, Nt/N for the controls, n_c = cases + controls of the cell),
+//   MI = H(G) + H(Y) - H(G,Y) = sum over cells and classes of D(x, y) / (N ln 2),
+//   D(x, y) = x ln(x/y) - x + y >= 0
+// (scoring.hpp:9-16; the -x + y parts add up to zero over a table).  The
+// entropy form subtracts sums of 2-5 bits to leave an MI near 1e-4 bits on
+// data without association, so its FP32 rounding is 10^4 times the score;
+// every D is >= 0, so this sum has no cancellation and keeps the relative
+// error of its terms.  With d = x - y (one fma, rounded once), e = d / y and
+// h(r) = r ln r - r + 1, D = y h(1 + e):
+//  * |e| <= 1/4: h(1 + e) = e^2 sum_{k>=2} (-e)^(k-2) / (k (k - 1)), summed
+//    to k = 12 (truncation below 6e-9 relative), so D = d e s;
+//  * otherwise y (r ln r + 1 - r), r = 1 + e, ln r from v_log_f32;
+//  * x = 0: D = y; n_c = 0: D = 0.
+// The host measures this term's relative error against FP64 over the cells
+// a search can produce (k_term_error) and sets the candidate margin from it.
+__device__ __forceinline__ float mi_dterm(uint32_t x, uint32_t n, float q)
 {
-    float sG = 0.f, cG = 0.f, sGY = 0.f, cGY = 0.f;
-    for (int i = 0; i < ncells; i++) {
-        kahan_add(sGY, cGY, plogp(cas[i], invN));
-        kahan_add(sGY, cGY, plogp(ctl[i], invN));
-        kahan_add(sG, cG, plogp(cas[i] + ctl[i], invN));
+#pragma clang fp contract(off)
+    if (n == 0u) return 0.f;
+    const float nf = (float)n;
+    const float y = q * nf;
+    if (x == 0u) return y;
+    const float d = fmaf(-q, nf, (float)x);
+    const float e = d * __builtin_amdgcn_rcpf(y);
+    if (fabsf(e) <= 0.25f) {
+        const float t = -e;
+        float s = 1.f / 132.f;
+        s = fmaf(s, t, 1.f / 110.f);
+        s = fmaf(s, t, 1.f / 90.f);
+        s = fmaf(s, t, 1.f / 72.f);
+        s = fmaf(s, t, 1.f / 56.f);
+        s = fmaf(s, t, 1.f / 42.f);
+        s = fmaf(s, t, 1.f / 30.f);
+        s = fmaf(s, t, 1.f / 20.f);
+        s = fmaf(s, t, 1.f / 12.f);
+        s = fmaf(s, t, 1.f / 6.f);
+        s = fmaf(s, t, 0.5f);
+        return (d * e) * s;
     }
-    // H(G) = -sG, H(G,Y) = -sGY
-    return (sGY - sG) + hY;
+    const float r = 1.f + e;
+    const float lr = __builtin_amdgcn_logf(r) * LN2_F;
+    return y * fmaf(r, lr, 1.f - r);
+}
+
+__device__ __forceinline__ float mi_f32(const uint32_t *cas, const uint32_t *ctl, int ncells, const float (&q)[2],
+                                        float scale)
+{
+    float s = 0.f;
+    for (int i = 0; i < ncells; i++) {
+        const uint32_t n = cas[i] + ctl[i];
+        s += mi_dterm(cas[i], n, q[0]) + mi_dterm(ctl[i], n, q[1]);
+    }
+    return s * scale;
 }
 
 // Lower bound of a table's badness (K2, or -MI), from
@@ -237,7 +272,7 @@ __device__ __forceinline__ float badness_f32(const ScoreCtx &sc, const uint32_t 
                                             const float *s_lnf)
 {
     if (sc.score == SCORE_K2) return k2_f32(cas, ctl, sc.ncells, sc.invN, sc.k0, sc.c, sc.k2_total, s_lnf);
-    return -mi_f32(cas, ctl, sc.ncells, sc.invN, sc.hY);
+    return -mi_f32(cas, ctl, sc.ncells, sc.mi_q, sc.mi_scale);
 }
 
 // ---------------------------------------------------------------- FP64 --

@@ -936,27 +936,69 @@ __global__ void k_init_many(unsigned long long *const *queue, unsigned *const *b
     if (i < nq) batch_queues[i] = 0ull;
 }
 
-// Largest error of the FP32 score terms over every argument a search can
-// reach, against the FP64 tables.  K2: |k2_term(m) - (T64[m] - c m)| for
-// m in [0, N+1].  MI: |plogp(m) - (m/N) L64[m]| * N/m for m in [1, N], the
-// error relative to p, so that a sum of terms whose p add up to P errs by at
-// most P times this.  *out holds the bits of a non-negative double, which
-// atomicMax on the bits orders correctly.
+// FP64 value of mi_dterm's D(x, q n) for the counts as mi_dterm sees them
+// (x, n rounded to FP32, exact below 2^24) and the same FP32 share q: the
+// series for |e| <= 1/4 (to k = 30, below 1e-18 relative), x ln(x/y) - d
+// otherwise, so that no cancellation limits the reference.
+__device__ double mi_dterm_f64(double x, double n, float q)
+{
+    if (n == 0.0) return 0.0;
+    const double y = (double)q * n;
+    if (x == 0.0) return y;
+    const double d = x - y, e = d / y;
+    if (fabs(e) <= 0.25) {
+        double s = 0.0;
+        for (int k = 30; k >= 2; k--) s = s * -e + 1.0 / ((double)k * (double)(k - 1));
+        return d * e * s;
+    }
+    return x * log(x / y) - d;
+}
+
+// Largest error of the FP32 score terms over the arguments a search can
+// reach, against FP64.  K2: |k2_term(m) - (T64[m] - c m)| for every m in
+// [0, N+1] (absolute).  MI: the relative error |D32 - D64| / D64 of
+// mi_dterm (against its own inputs: counts past 2^24 round to FP32 first,
+// which the host's bound covers apart) for both classes' shares q0, q1 and
+// every cell size n in [1, N]
+// (every 2^(k)-th n past 2^22 samples, k the smallest that leaves 2^22 of
+// them), at the counts x where its two forms meet (|e| = 1/4), around the
+// expected count, at 0, n and a spread in between; the terms are >= 0, so
+// a table's MI errs by at most this relative error plus the summation's.
+// *out holds the bits of a non-negative double, which atomicMax on the bits
+// orders correctly.
 __global__ void k_term_error(int score, int N, float invN, float k0, float c32, double c, const double *t64,
-                             unsigned long long *out)
+                             float q0, float q1, unsigned long long *out)
 {
     __shared__ float s_lnf[LNF_SMALL];
     load_lnf(s_lnf);
     __syncthreads();
     double e = 0.0;
-    const int hi = score == SCORE_K2 ? N + 1 : N;
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m <= hi; m += gridDim.x * blockDim.x) {
-        if (score == SCORE_K2) {
+    if (score == SCORE_K2) {
+        for (int m = blockIdx.x * blockDim.x + threadIdx.x; m <= N + 1; m += gridDim.x * blockDim.x) {
             const double exact = t64[m] - c * (double)m;
             e = fmax(e, fabs((double)k2_term((uint32_t)m, invN, k0, c32, s_lnf) - exact));
-        } else if (m >= 1) {
-            const double pm = (double)m / (double)N;
-            e = fmax(e, fabs((double)plogp((uint32_t)m, invN) - pm * t64[m]) / pm);
+        }
+    } else {
+        int step = 1;
+        while ((N - 1) / step + 1 > (1 << 22)) step *= 2;
+        for (long long i = blockIdx.x * blockDim.x + threadIdx.x; 1 + i * step <= N; i += gridDim.x * blockDim.x) {
+            const uint32_t n = (uint32_t)(1 + i * step);
+            for (int cls = 0; cls < 2; cls++) {
+                const float q = cls ? q1 : q0;
+                const double y = (double)q * (double)n;
+                const double at[8] = {0.0, (double)n, y, 0.75 * y, 1.25 * y, 0.5 * y, 2.0 * y, 0.1 * y};
+                for (int a = 0; a < 8; a++) {
+                    const long long x0 = (long long)floor(at[a]);
+                    for (int j = (a < 2 ? 0 : -3); j <= (a < 2 ? 0 : 3); j++) {
+                        const long long x = x0 + j;
+                        if (x < 0 || x > (long long)n) continue;
+                        const double ref = mi_dterm_f64((double)(float)x, (double)(float)n, q);
+                        const double got = (double)mi_dterm((uint32_t)x, n, q);
+                        if (ref > 0.0) e = fmax(e, fabs(got - ref) / ref);
+                        else if (got != 0.0) e = INFINITY;
+                    }
+                }
+            }
         }
     }
 #pragma unroll

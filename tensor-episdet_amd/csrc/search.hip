@@ -152,6 +152,14 @@ int ensure_lnfact()
     return EPISDET_OK;
 }
 
+// FP32 shares of the classes for the MI terms (mi_dterm): Nc/N rounded, and
+// 1 minus that rounded.
+void mi_shares(const episdet_dataset *ds, float q[2])
+{
+    q[0] = (float)((double)ds->N[0] / (double)(ds->N[0] + ds->N[1]));
+    q[1] = 1.f - q[0];
+}
+
 int ensure_tables(episdet_dataset *ds, int score)
 {
     ScoreTables &t = ds->tab[score];
@@ -182,8 +190,10 @@ int ensure_tables(episdet_dataset *ds, int score)
     const unsigned long long zero = 0ull;
     HIPCHK(hipMemcpy(d_err, &zero, sizeof zero, hipMemcpyHostToDevice));
     const int blocks = std::min(1024, (len + 255) / 256);
-    hipLaunchKernelGGL(k_term_error, dim3(blocks), dim3(256), 0, ds->stream, score, N, t.invN, t.k0, (float)t.shift_c, t.shift_c, t.t64,
-                       d_err);
+    float q[2];
+    mi_shares(ds, q);
+    hipLaunchKernelGGL(k_term_error, dim3(blocks), dim3(256), 0, ds->stream, score, N, t.invN, t.k0,
+                       (float)t.shift_c, t.shift_c, t.t64, q[0], q[1], d_err);
     HIPCHK(hipGetLastError());
     unsigned long long bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, d_err, sizeof bits, hipMemcpyDeviceToHost, ds->stream));
@@ -219,7 +229,8 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
     sc.c = (float)t.shift_c;
     sc.invN = t.invN;
     const double qc = (double)ds->N[0] / N, qt = (double)ds->N[1] / N;
-    sc.hY = (float)(-qc * std::log2(qc) - qt * std::log2(qt));
+    mi_shares(ds, sc.mi_q);
+    sc.mi_scale = (float)(1.0 / ((double)N * std::log(2.0)));
     double eps;
     if (score == EPISDET_SCORE_K2) {
         sc.k2_total = (float)(ncells * t.shift_c);
@@ -231,12 +242,19 @@ ScoreCtx make_ctx(const episdet_dataset *ds, int score, int order)
         eps = ncells * (3.0 * t.term_err + 2.0 * ulp32(3.0 * T)) + 2.0 * u * sum_bound + ulp32(sum_bound) +
               ulp32(ncells * t.shift_c);
     } else {
-        // Terms p log2 p: the probabilities of each of H(G,Y) and H(G) add up
-        // to 1, so the terms err by at most 2 term_err in total; the
-        // compensated sums and the final additions add a few ulps of values
-        // bounded by log2(2 ncells).
-        const double hmax = std::log2(2.0 * ncells);
-        eps = 2.0 * t.term_err + u * (3.0 * ncells * 0.531 * 2.0 + 8.0 * hmax) + 4.0 * u;
+        // The 2 ncells terms D >= 0 each err by at most term_err relative
+        // (measured), their sum by 2 ncells roundings, and the scale by two
+        // more, so the FP32 MI errs by at most (term_err + (2 ncells + 3) u)
+        // times the MI, which never exceeds H(Y).  Rounding the classes'
+        // shares to FP32 moves the MI by N KL(pi || fl(pi)) / (N ln 2), of
+        // order u^2: the absolute 4u covers it.  Counts of 2^24 and more
+        // round to FP32 (x and n by u relative each), which moves D by
+        // |ln(x/y)| x u + |d| u <= (2 D + x + |d|) u; the x and |d| of a
+        // table add up to at most 3N, so the MI moves by at most
+        // 2u MI + 3u / ln 2 in bits: 16u covers both.
+        const double hY = -qc * std::log2(qc) - qt * std::log2(qt);
+        eps = hY * (t.term_err + (2.0 * ncells + 3.0) * u) * (1.0 + 1e-6) + 4.0 * u +
+              (N >= (1 << 24) ? 16.0 * u : 0.0);
     }
     sc.margin = (float)(2.0 * eps);
     // Badness lower bound (bound_exceeds): lb_max - lb_scale * S, and
