@@ -23,7 +23,10 @@ null_data  = the same search on the same shape with no planted set (the
              step, reported beside the headline
 plain      = the reference-shaped round loop (count kernel -> int32 tables in
              HBM -> FP64 objective kernel), timed on its first R rounds and
-             scaled by the round count (all rounds have identical work)
+             scaled by the round count (all rounds have identical work), with
+             its rounds counted by the FP4 MFMA kernel (mfma_count: fused /
+             plain is then the gain of fusion alone) and on the VALU
+             (valu_count)
 cpu        = the OpenMP AND+popcount oracle on a contiguous slice of the same
              search (sets whose largest index lies in a stated range), on
              every host CPU this process may use, at most the OMP_NUM_THREADS
@@ -281,12 +284,20 @@ def main():
 
     plain = None
     if not args.no_plain and order == 3:
-        rr = ds.run(order, score, path=episdet.PATH_PLAIN, plain_rounds=args.plain_rounds)
-        total_rounds = rr["rounds"]
-        est = rr["kernel_ms"] / 1e3 * total_rounds / rr["rounds_run"]
-        plain = {"value": sets * N / est, "rounds_timed": rr["rounds_run"], "rounds_total": total_rounds,
-                 "seconds_est": est, "dtype": "f64 scoring", "speedup_fused": (sets * N / est) and value / (sets * N / est)}
-        log(f"plain: {rr['rounds_run']} rounds in {rr['kernel_ms']:.1f} ms -> {est:.1f} s estimated")
+        # the same round loop with each of its two count kernels: FP4 MFMA
+        # (the fused path's counting engine, so fused / plain is the gain of
+        # fusion alone) and AND + v_bcnt on the VALU
+        plain = {}
+        for name, tile in (("mfma_count", 1), ("valu_count", 0)):
+            if tile == 1 and ran != "mfma":
+                continue
+            rr = ds.run(order, score, path=episdet.PATH_PLAIN, tile=tile, plain_rounds=args.plain_rounds)
+            total_rounds = rr["rounds"]
+            est = rr["kernel_ms"] / 1e3 * total_rounds / rr["rounds_run"]
+            plain[name] = {"value": sets * N / est, "rounds_timed": rr["rounds_run"], "rounds_total": total_rounds,
+                           "seconds_est": est, "dtype": "fp4 count" if tile == 1 else "u32 count",
+                           "scoring": "f64", "speedup_fused": value / (sets * N / est)}
+            log(f"plain ({name}): {rr['rounds_run']} rounds in {rr['kernel_ms']:.1f} ms -> {est:.1f} s estimated")
 
     null = None
     if not args.no_null and pre["planted"] and order == 3:

@@ -105,7 +105,10 @@ typedef struct {
                              sequence), FUSED_MFMA 2-way 0-1
                              (0: 128 x 64 units; 1: 64 x 64, classes in
                              sequence), FUSED_VALU 3-way 0-3, FUSED_VALU 2-way
-                             0-2; PLAIN ignores it                               */
+                             0-2; PLAIN: the count kernel of its rounds, 0 =
+                             AND + v_bcnt on the VALU, 1 = 3-way only, FP4 MFMA
+                             (the 64 x 64 kernel of FUSED_MFMA tile 0 writing
+                             its counts; 16 rounds per launch)                   */
     int plain_rounds;     /* PLAIN only: stop after this many rounds (0 = all);
                              used to time a contiguous slice of the round loop   */
     int candidate_cap;    /* FP32 candidates kept per launch for FP64 re-scoring;

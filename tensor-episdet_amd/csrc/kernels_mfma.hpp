@@ -143,6 +143,13 @@ struct BatchTri {
     unsigned long long total;
     unsigned long long *queue;
     unsigned sh = 1;                        // class-3 operand shift (see m2_aop)
+    // MODE_COUNTS (k_triplets_mfma2, the plain path on the matrix cores):
+    // `rounds` rounds (X0 + r, Y block yb, Z block zb of 256) per launch,
+    // 16 workgroups each; round r writes its AND-counts to counts + r * slot
+    // as [class][256 Y][256 Z][8] (the layout k_objective_plain reads)
+    int X0 = 0, yb = 0, zb = 0, rounds = 0;
+    uint32_t *counts = nullptr;
+    size_t slot = 0;
 };
 
 template <int MODE>
@@ -463,28 +470,41 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
     for (int cb = 0; cb < 4; cb++) zr[cb] = M2_ZROW + 2 * (cb * 16 + (i >> 1)) + (i & 1);
 
     int g = 0, j = 0;
-    for (;;) {
-        if (tid == 0) s_u = atomicAdd(bt.queue, 1ull);
-        __syncthreads();
-        const unsigned long long uu = s_u;
-        if (uu >= bt.total) break;
-        // job of this unit; a workgroup's units only increase
-        int jn = j;
-        while (jn + 1 < bt.njobs && bt.job_start[jn + 1] <= uu) jn++;
-        if (jn != j) {
-            j = jn;
-            g = 0;
+    for (int iter = 0;; iter++) {
+        int X, y0, z0;
+        if constexpr (MODE == MODE_COUNTS) {
+            // one 64 x 64 tile of round blockIdx.x / 16, no queue
+            if (iter > 0) break;
+            const int r = (int)blockIdx.x >> 4, tl = (int)blockIdx.x & 15;
+            if (r >= bt.rounds) break;
+            X = bt.X0 + r;
+            y0 = bt.yb * 256 + (tl >> 2) * M2_TY;
+            z0 = bt.zb * 256 + (tl & 3) * M2_TZ;
+        } else {
+            if (tid == 0) s_u = atomicAdd(bt.queue, 1ull);
+            __syncthreads();
+            const unsigned long long uu = s_u;
+            if (uu >= bt.total) break;
+            // job of this unit; a workgroup's units only increase
+            int jn = j;
+            while (jn + 1 < bt.njobs && bt.job_start[jn + 1] <= uu) jn++;
+            if (jn != j) {
+                j = jn;
+                g = 0;
+            }
+            const unsigned long long u = uu - bt.job_start[j];
+            const TriArgs &pj = bt.jobs[j];
+            g = next_group(pj.gstart, pj.ngroups, u, g);
+            X = (int)(u - pj.gstart[g]);
+            const int2 gt = pj.groups[g];
+            y0 = gt.x * M2_TY;
+            z0 = gt.y * M2_TZ;
         }
         const TriArgs &p = bt.jobs[j];
-        const unsigned long long u = uu - bt.job_start[j];
         const uint32_t *base = cls ? p.p1 : p.p0;
         const int W = cls ? p.W1 : p.W0;
         const int nch_mine = W / M2_KW;
         const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / M2_KW;
-        g = next_group(p.gstart, p.ngroups, u, g);
-        const int X = (int)(u - p.gstart[g]);
-        const int2 gt = p.groups[g];
-        const int y0 = gt.x * M2_TY, z0 = gt.y * M2_TZ;
         const i32x4_t rsrc = m2_rsrc(base, cls ? p.bytes1 : p.bytes0);
         unsigned roff[M2_LOADS_PER_WAVE];
         m2_row_offsets(W, X, y0, z0, wy, lane, roff);
@@ -523,7 +543,7 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
             __syncthreads();
         }
 
-        const Singles sx = ld_global(p.singles, X);
+        const Singles sx = MODE == MODE_COUNTS ? Singles{0, 0, 0, 0} : ld_global(p.singles, X);
         float bad[8];
         unsigned long long packed[8];
         unsigned long long hsum = 0;
@@ -549,7 +569,8 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
                 const int Y = y0 + yl, Z = z0 + zl;
                 bad[m] = INFINITY;
                 packed[m] = (unsigned long long)X | ((unsigned long long)Y << 21) | ((unsigned long long)Z << 42);
-                if (!(X < Y && Y < Z && Z < p.n)) continue;
+                // the plain path's objective kernel checks the set itself
+                if (MODE != MODE_COUNTS && !(X < Y && Y < Z && Z < p.n)) continue;
                 const int twy = yl >> 4, trb = (yl >> 3) & 1, y8 = yl & 7, tq = y8 >> 1, th = y8 & 1;
                 const int tcr = (T & 31) >> 4, z16 = T & 15;
                 const int t4 = trb * 2 + tcr;
@@ -562,6 +583,17 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
                         n8c[ij * 2 + kk] = smem[(0 * 4 + twy) * 4 * 16 * 64 + off];
                         n8t[ij * 2 + kk] = smem[(1 * 4 + twy) * 4 * 16 * 64 + off];
                     }
+                if constexpr (MODE == MODE_COUNTS) {
+                    const int yr = Y - bt.yb * 256, zr = Z - bt.zb * 256;
+                    uint32_t *rc = bt.counts + (size_t)(X - bt.X0) * bt.slot;
+                    uint4 *o0 = (uint4 *)(rc + ((size_t)(0 * 256 + yr) * 256 + zr) * 8);
+                    uint4 *o1 = (uint4 *)(rc + ((size_t)(1 * 256 + yr) * 256 + zr) * 8);
+                    o0[0] = make_uint4(n8c[0], n8c[1], n8c[2], n8c[3]);
+                    o0[1] = make_uint4(n8c[4], n8c[5], n8c[6], n8c[7]);
+                    o1[0] = make_uint4(n8t[0], n8t[1], n8t[2], n8t[3]);
+                    o1[1] = make_uint4(n8t[4], n8t[5], n8t[6], n8t[7]);
+                    continue;
+                }
                 const Singles sy = ld_global(p.singles, Y);
                 uint32_t cas[27], ctl[27];
                 triplet_tables(p, X, Y, Z, sx, sy, n8c, n8t, cas, ctl);

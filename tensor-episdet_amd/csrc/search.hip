@@ -35,6 +35,7 @@ using namespace episdet;
 namespace {
 
 constexpr int BLOCK = 256;            // the reference's BLOCK_SIZE (Makefile:2)
+constexpr int PLAIN_GROUP = 16;       // rounds per FP4 count launch of the plain path
 constexpr int NUM_STREAMS = 8;        // the reference's NUM_STREAMS (Makefile:2)
 constexpr unsigned DEFAULT_CAP = 1u << 20;
 
@@ -1341,11 +1342,19 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     CHK(ensure_tables(ds, score));
     if (order == 3) CHK(ensure_pairs(ds));
     const int n = ds->n, nb = (n + BLOCK - 1) / BLOCK;
+    // tile 1: the 3-way count kernel on the matrix cores (the 64 x 64 FP4
+    // kernel writing its AND-counts), PLAIN_GROUP rounds of one (Y block,
+    // Z block) and consecutive X per launch so that a launch fills the GPU
+    // (16 workgroups per round); tile 0: the VALU count kernel, one round
+    // per launch, as the reference's loop
+    const bool mfma = o->tile == 1;
+    if (o->tile != 0 && !(mfma && order == 3 && mfma_ok(ds))) return EPISDET_ERR_INVALID;
+    const int group = mfma ? PLAIN_GROUP : 1;
     const size_t slot = (size_t)2 * 256 * 256 * (order == 3 ? 8 : 4);
     DevList dl;
     uint32_t *counts = nullptr;
     unsigned long long *blk = nullptr, *sbest = nullptr, *flag = nullptr;
-    CHK(dl.alloc(&counts, slot * NUM_STREAMS));
+    CHK(dl.alloc(&counts, slot * NUM_STREAMS * group));
     CHK(dl.alloc(&blk, (size_t)NUM_STREAMS * 2 * 512));      // [stream][parity][256][2]
     CHK(dl.alloc(&sbest, (size_t)NUM_STREAMS * 2));          // [stream][2]
     CHK(dl.alloc(&flag, 1));
@@ -1364,6 +1373,7 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     const int plain_tile = 1;   // 32 x 32 (Y, Z) per workgroup: 64 workgroups per round
     TriArgs ta;
     if (order == 3) fill_tri_common(ds, ta);
+    if (mfma) CHK(upload_single(ds, ta, 0));    // the planes and sizes the count kernel reads
     PlainObjArgs po;
     std::memset(&po, 0, sizeof po);
     po.singles = ds->singles;
@@ -1379,20 +1389,8 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     const int grid_count = (256 / (16 * t.A)) * (256 / (16 * t.B));
     long long rounds = 0, per_stream[NUM_STREAMS] = {};
     const long long limit = o->plain_rounds > 0 ? o->plain_rounds : -1;
-    // one round: count kernel, then the objective kernel, on stream s
-    auto round = [&](int X, int yb, int zb) -> int {
-        const int s = (int)(rounds % NUM_STREAMS);
-        uint32_t *c = counts + slot * s;
-        if (order == 3) {
-            ta.X = X;
-            ta.yb = yb;
-            ta.zb = zb;
-            ta.counts = c;
-            CHK(dispatch_tri<MODE_COUNTS>(ds, plain_tile, ta, grid_count, ss.st[s]));
-        } else {
-            hipLaunchKernelGGL(k_pairs_counts, dim3(16), dim3(256), 0, ss.st[s], ds->planes[0], ds->W[0],
-                               ds->planes[1], ds->W[1], yb, zb, c);
-        }
+    // the objective kernel of one round whose counts are at c, on stream s
+    auto objective = [&](int X, int yb, int zb, const uint32_t *c, int s) -> int {
         const long long k = per_stream[s];
         po.counts = c;
         po.X = X;
@@ -1407,8 +1405,61 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
         rounds++;
         return EPISDET_OK;
     };
+    // one round: count kernel, then the objective kernel, on stream s
+    auto round = [&](int X, int yb, int zb) -> int {
+        const int s = (int)(rounds % NUM_STREAMS);
+        uint32_t *c = counts + slot * s;
+        if (order == 3) {
+            ta.X = X;
+            ta.yb = yb;
+            ta.zb = zb;
+            ta.counts = c;
+            CHK(dispatch_tri<MODE_COUNTS>(ds, plain_tile, ta, grid_count, ss.st[s]));
+        } else {
+            hipLaunchKernelGGL(k_pairs_counts, dim3(16), dim3(256), 0, ss.st[s], ds->planes[0], ds->W[0],
+                               ds->planes[1], ds->W[1], yb, zb, c);
+        }
+        return objective(X, yb, zb, c, s);
+    };
+    // cnt rounds (X0 .. X0 + cnt - 1, yb, zb): one FP4 count launch, then
+    // their objective kernels, on one stream
+    long long launches = 0;
+    auto mfma_rounds = [&](int X0, int cnt, int yb, int zb) -> int {
+        const int s = (int)(launches++ % NUM_STREAMS);
+        uint32_t *c = counts + slot * group * s;
+        BatchTri bt;
+        bt.jobs = (const TriArgs *)ds->jobargs;
+        bt.job_start = ds->jobstart;
+        bt.njobs = 1;
+        bt.total = 0;
+        bt.queue = nullptr;
+        bt.X0 = X0;
+        bt.yb = yb;
+        bt.zb = zb;
+        bt.rounds = cnt;
+        bt.counts = c;
+        bt.slot = slot;
+        hipLaunchKernelGGL((k_triplets_mfma2<MODE_COUNTS>), dim3(16 * cnt), dim3(512), 0, ss.st[s], bt);
+        HIPCHK(hipGetLastError());
+        for (int r = 0; r < cnt; r++) CHK(objective(X0 + r, yb, zb, c + slot * r, s));
+        return EPISDET_OK;
+    };
     bool stop = false;
-    if (order == 3) {
+    if (mfma) {
+        for (int yb = 0; yb < nb && !stop; yb++)
+            for (int zb = yb; zb < nb && !stop; zb++) {
+                const int xend = std::min((yb + 1) * BLOCK, n);
+                for (int X0 = 0; X0 < xend; X0 += group) {
+                    long long cnt = std::min(group, xend - X0);
+                    if (limit >= 0) cnt = std::min(cnt, limit - rounds);
+                    if (cnt <= 0) {
+                        stop = true;
+                        break;
+                    }
+                    CHK(mfma_rounds(X0, (int)cnt, yb, zb));
+                }
+            }
+    } else if (order == 3) {
         for (int yb = 0; yb < nb && !stop; yb++)
             for (int X = 0; X < std::min((yb + 1) * BLOCK, n) && !stop; X++)
                 for (int zb = yb; zb < nb; zb++) {
@@ -1464,6 +1515,7 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     unpack(order, bp, res->idx);
     res->candidates = 0;
     res->path = EPISDET_PATH_PLAIN;
+    res->tile = mfma ? 1 : 0;
     return EPISDET_OK;
 }
 
