@@ -53,3 +53,30 @@ def test_cli_errors(tmp_path):
     assert p.returncode == 1 and "Usage" in p.stderr
     p = subprocess.run([exe, str(tmp_path / "nope.txt")], capture_output=True, text=True, timeout=60)
     assert p.returncode == 1 and "does not exist" in p.stderr
+
+
+SHIM_BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shim", "bin")
+
+
+@pytest.mark.parametrize("variant,order,score", [("triplets.k2", 3, O.K2), ("pairs.mi", 2, O.MI)])
+def test_reference_entry_point_through_shim(variant, order, score):
+    """A C program written against EpistasisDetectionSearch
+    (search-triplets.hpp:4), linked with tests/shim/episdet_shim.cpp (the
+    binding INTEGRATION.md shows) instead of the reference's CUDA objects:
+    the library prints the reference's "Outer loop iteration i out of nb"
+    lines itself (search-triplets.cu:164, search-pairs.cu:139), before the
+    program's own output, and the winner is the oracle's."""
+    exe = os.path.join(SHIM_BIN, f"shim_main.{variant}")
+    n, Nc, Nt, seed = 600, 1500, 1300, 33
+    print("shim", exe, flush=True)
+    p = subprocess.run([exe, str(n), str(Nc), str(Nt), str(seed)], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    lines = p.stdout.splitlines()
+    assert lines[:3] == [f"Outer loop iteration {i} out of 3" for i in range(3)]
+    syn = episdet.generate(n, Nc, Nt, seed=seed, planted_sets=1, planted_order=order)
+    o = O.search(syn.cases, syn.controls, Nc, Nt, order, score)
+    want = ", ".join(str(v) for v in o["idx"][:order])
+    assert lines[3].split("\t->\t")[1].startswith("{" + want + "}: "), lines[3]
+    assert float(lines[3].rsplit(": ", 1)[1]) == pytest.approx(o["score"], rel=1e-6, abs=5e-7)
+    assert lines[4] == f"Num. of rounds processed: {episdet.reference_rounds(order, n)}"
+    assert lines[5] == "planted: " + " ".join(str(v) for v in syn.planted[0])
