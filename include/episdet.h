@@ -43,8 +43,10 @@ enum episdet_status {
                                      resolved (not returned by this version: an
                                      overflow reruns the search in windows)     */
     EPISDET_ERR_INTERNAL = 6,
-    EPISDET_ERR_GENOTYPES = 7     /* a sample is set in both planes of a SNP: the
+    EPISDET_ERR_GENOTYPES = 7,    /* a sample is set in both planes of a SNP: the
                                      input encodes no genotype for it             */
+    EPISDET_ERR_SCRATCH = 8       /* episdet_run_noalloc: the search needs more
+                                     device memory than its static scratch holds */
 };
 
 enum episdet_score { EPISDET_SCORE_K2 = 0, EPISDET_SCORE_MI = 1 };
@@ -160,6 +162,28 @@ typedef struct {
 } episdet_result;
 
 int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result *res);
+
+/* episdet_run without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same options, result and answer as episdet_run on the same
+ * dataset, except:
+ *  - its per-search device memory (control block, argument block, 2^18
+ *    candidate slots, work-unit groups, and the FP64 score table and 3-way
+ *    pair table when the dataset has not built them in an earlier
+ *    episdet_run) is a static 64 MiB array that every device the library
+ *    runs on holds until the process ends; a search that needs more returns
+ *    EPISDET_ERR_SCRATCH before touching the device (without the dataset's
+ *    own tables: 3-way up to about 1400 SNPs, any order up to about 7.5M
+ *    samples);
+ *  - PLAIN, rebuild and autotune are refused (EPISDET_ERR_INVALID): they
+ *    allocate buffers of their own;
+ *  - no seeding pre-pass (result.seeded = 0): it runs a search of its own.
+ * Calls share the array of their device, which keeps the last call's
+ * contents, so calls that overlapped on one device would race: the library
+ * runs the calls of a process one at a time (one lock).  Dataset creation
+ * itself allocates, as for episdet_run. */
+int episdet_run_noalloc(episdet_dataset *ds, const episdet_options *opt, episdet_result *res);
 
 /* ---- autotuner (no reference counterpart: the reference's BLOCK_SIZE and
  * NUM_STREAMS are compile-time constants, search-triplets.cu:4-5) ----

@@ -83,6 +83,80 @@ static_assert(offsetof(Ctl, cand_over) == offsetof(Ctl, cand_count) + sizeof(uns
 
 }  // namespace
 
+// Device memory of the no-allocation search (episdet_run_noalloc): control
+// block, argument block, candidate slots, work-unit groups and, when the
+// dataset has not built them yet, the FP64 score table and the 3-way pair
+// table.  The runtime reserves it on every device when the module loads and
+// keeps it until the process ends.  In host code the name stands for a
+// host-side shadow, not a device address: host code uses only the address
+// hipGetSymbolAddress returns for the current device (scratch_device).
+constexpr size_t SCRATCH_BYTES = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+namespace {
+
+constexpr unsigned SCRATCH_CAND = 1u << 18;     // candidate slots of a no-allocation search
+
+// The regions of `scratch` one no-allocation call uses (device addresses),
+// and its two timing events.  Set for the duration of the call on its host
+// thread (t_scr); every device allocation site of the search checks it.
+struct ScratchPlan {
+    Ctl *ctl = nullptr;
+    void *jobargs = nullptr;
+    unsigned long long *jobstart = nullptr;
+    unsigned long long *err = nullptr;
+    unsigned long long *cand = nullptr;
+    float *cand_s32 = nullptr;
+    double *cand_s64 = nullptr;
+    unsigned long long *gstart = nullptr;
+    int2 *groups = nullptr;
+    size_t max_groups = 0;
+    double *t64 = nullptr;          // when the dataset has no FP64 table for the score
+    uint4 *pairs = nullptr;         // when a 3-way dataset has no pair table
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+thread_local const ScratchPlan *t_scr = nullptr;
+
+// Every device allocation of the search goes through here.  Inside a
+// no-allocation call it fails instead of allocating (the call then returns
+// an error), so such a call can never reach hipMalloc.
+hipError_t dev_malloc(void **p, size_t bytes)
+{
+    if (t_scr) {
+        *p = nullptr;
+        return hipErrorNotSupported;
+    }
+    return hipMalloc(p, bytes);
+}
+
+// Two timing events: the no-allocation call's own (made once per device at
+// its set-up), or two new ones destroyed with the Timer.
+struct Timer {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool own = false;
+    int make()
+    {
+        if (t_scr) {
+            e0 = t_scr->ev[0];
+            e1 = t_scr->ev[1];
+            return EPISDET_OK;
+        }
+        own = true;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        return EPISDET_OK;
+    }
+    ~Timer()
+    {
+        if (own) {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    }
+};
+
+}  // namespace
+
 struct episdet_dataset {
     int dev = 0;
     int n = 0, nrows = 0, N[2] = {0, 0}, W[2] = {0, 0};
@@ -122,6 +196,7 @@ int check_device(int *cu)
 int ensure_cand(episdet_dataset *ds, unsigned cap)
 {
     if (ds->cand_cap >= cap) return EPISDET_OK;
+    if (t_scr) return EPISDET_ERR_SCRATCH;           // the slots of `scratch` are all there is
     if (ds->cand) (void)hipFree(ds->cand);
     if (ds->cand_s32) (void)hipFree(ds->cand_s32);
     if (ds->cand_s64) (void)hipFree(ds->cand_s64);
@@ -129,9 +204,9 @@ int ensure_cand(episdet_dataset *ds, unsigned cap)
     ds->cand_s32 = nullptr;
     ds->cand_s64 = nullptr;
     ds->cand_cap = 0;
-    if (hipMalloc(&ds->cand, sizeof(unsigned long long) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
-    if (hipMalloc(&ds->cand_s32, sizeof(float) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
-    if (hipMalloc(&ds->cand_s64, sizeof(double) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (dev_malloc((void **)&ds->cand, sizeof(unsigned long long) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (dev_malloc((void **)&ds->cand_s32, sizeof(float) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (dev_malloc((void **)&ds->cand_s64, sizeof(double) * cap) != hipSuccess) return EPISDET_ERR_ALLOC;
     ds->cand_cap = cap;
     return EPISDET_OK;
 }
@@ -183,11 +258,15 @@ int ensure_tables(episdet_dataset *ds, int score)
         h64[0] = 0.0;
         for (int m = 1; m <= N; m++) h64[m] = std::log2((double)m / (double)N);
     }
-    if (hipMalloc(&t.t64, sizeof(double) * len) != hipSuccess) return EPISDET_ERR_ALLOC;
+    // inside a no-allocation call the table and the error word live in
+    // `scratch` (the caller restores the dataset's tables afterwards)
+    if (t_scr) t.t64 = t_scr->t64;
+    else if (dev_malloc((void **)&t.t64, sizeof(double) * len) != hipSuccess) return EPISDET_ERR_ALLOC;
+    if (!t.t64) return EPISDET_ERR_INTERNAL;
     HIPCHK(hipMemcpy(t.t64, h64.data(), sizeof(double) * len, hipMemcpyHostToDevice));
     // error of the FP32 terms over every argument this dataset can produce
-    unsigned long long *d_err = nullptr;
-    if (hipMalloc(&d_err, sizeof(unsigned long long)) != hipSuccess) return EPISDET_ERR_ALLOC;
+    unsigned long long *d_err = t_scr ? t_scr->err : nullptr;
+    if (!d_err && dev_malloc((void **)&d_err, sizeof(unsigned long long)) != hipSuccess) return EPISDET_ERR_ALLOC;
     const unsigned long long zero = 0ull;
     HIPCHK(hipMemcpy(d_err, &zero, sizeof zero, hipMemcpyHostToDevice));
     const int blocks = std::min(1024, (len + 255) / 256);
@@ -199,7 +278,7 @@ int ensure_tables(episdet_dataset *ds, int score)
     unsigned long long bits = 0;
     HIPCHK(hipMemcpyAsync(&bits, d_err, sizeof bits, hipMemcpyDeviceToHost, ds->stream));
     HIPCHK(hipStreamSynchronize(ds->stream));
-    (void)hipFree(d_err);
+    if (!t_scr) (void)hipFree(d_err);
     std::memcpy(&t.term_err, &bits, sizeof bits);
     if (!(t.term_err >= 0.0 && t.term_err < 1e3)) return EPISDET_ERR_INTERNAL;
     t.ready = true;
@@ -311,7 +390,7 @@ struct DevList {   // device allocations released together
     int alloc(T **dst, size_t n)
     {
         *dst = nullptr;
-        if (hipMalloc((void **)dst, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return EPISDET_ERR_ALLOC;
+        if (dev_malloc((void **)dst, sizeof(T) * std::max<size_t>(n, 1)) != hipSuccess) return EPISDET_ERR_ALLOC;
         ptrs.push_back((void *)*dst);
         return EPISDET_OK;
     }
@@ -634,7 +713,10 @@ int ensure_pairs(episdet_dataset *ds)
 {
     if (ds->pairs) return EPISDET_OK;
     const size_t bytes = (size_t)ds->n * ds->n * 2 * sizeof(uint4);
-    if (hipMalloc(&ds->pairs, bytes) != hipSuccess) {
+    if (t_scr) {
+        ds->pairs = t_scr->pairs;        // restored by the no-allocation call
+        if (!ds->pairs) return EPISDET_ERR_INTERNAL;
+    } else if (dev_malloc((void **)&ds->pairs, bytes) != hipSuccess) {
         ds->pairs = nullptr;
         return EPISDET_ERR_ALLOC;
     }
@@ -697,19 +779,16 @@ int rescore(episdet_dataset *ds, int order, int score, int count, double *ms)
     r.cand = ds->cand;
     r.count = count;
     r.out_s64 = ds->cand_s64;
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, ds->stream));
+    Timer tm;
+    CHK(tm.make());
+    HIPCHK(hipEventRecord(tm.e0, ds->stream));
     hipLaunchKernelGGL(k_rescore, dim3(count), dim3(64), 0, ds->stream, r);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(e1, ds->stream));
-    HIPCHK(hipEventSynchronize(e1));
+    HIPCHK(hipEventRecord(tm.e1, ds->stream));
+    HIPCHK(hipEventSynchronize(tm.e1));
     float f = 0;
-    HIPCHK(hipEventElapsedTime(&f, e0, e1));
+    HIPCHK(hipEventElapsedTime(&f, tm.e0, tm.e1));
     *ms = f;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     return EPISDET_OK;
 }
 
@@ -812,11 +891,13 @@ struct FusedPlan {
     int2 *d_groups = nullptr;
     unsigned long long nunits = 0;
     unsigned seed_key = 0;       // starting global bound (best32), 0: none (see seed_from_pairs)
+    bool own_groups = false;     // d_gstart / d_groups allocated here (not in `scratch`)
     FusedPlan() = default;
     FusedPlan(const FusedPlan &) = delete;
     FusedPlan &operator=(const FusedPlan &) = delete;
     ~FusedPlan()
     {
+        if (!own_groups) return;
         if (d_gstart) (void)hipFree(d_gstart);
         if (d_groups) (void)hipFree(d_groups);
     }
@@ -849,9 +930,17 @@ int plan_fused(episdet_dataset *ds, int order, int score, int path, int tile, Fu
             build_groups(ds->n, 16 * t.A, 16 * t.B, pl.groups, pl.gstart);
         }
         if (pl.groups.empty()) return EPISDET_ERR_INVALID;
-        if (hipMalloc(&pl.d_gstart, sizeof(unsigned long long) * pl.gstart.size()) != hipSuccess)
-            return EPISDET_ERR_ALLOC;
-        if (hipMalloc(&pl.d_groups, sizeof(int2) * pl.groups.size()) != hipSuccess) return EPISDET_ERR_ALLOC;
+        if (t_scr) {
+            if (pl.groups.size() > t_scr->max_groups) return EPISDET_ERR_INTERNAL;   // sized by the caller
+            pl.d_gstart = t_scr->gstart;
+            pl.d_groups = t_scr->groups;
+        } else {
+            pl.own_groups = true;
+            if (dev_malloc((void **)&pl.d_gstart, sizeof(unsigned long long) * pl.gstart.size()) != hipSuccess)
+                return EPISDET_ERR_ALLOC;
+            if (dev_malloc((void **)&pl.d_groups, sizeof(int2) * pl.groups.size()) != hipSuccess)
+                return EPISDET_ERR_ALLOC;
+        }
         HIPCHK(hipMemcpy(pl.d_gstart, pl.gstart.data(), sizeof(unsigned long long) * pl.gstart.size(),
                          hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(pl.d_groups, pl.groups.data(), sizeof(int2) * pl.groups.size(), hipMemcpyHostToDevice));
@@ -916,12 +1005,9 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
     if (u0) HIPCHK(hipMemcpyAsync(&ds->ctl->queue, &u0, sizeof u0, hipMemcpyHostToDevice, ds->stream));
     if (seed_key)
         HIPCHK(hipMemcpyAsync(&ds->ctl->best32, &seed_key, sizeof seed_key, hipMemcpyHostToDevice, ds->stream));
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return EPISDET_ERR_HIP;
-    }
+    Timer tm;
+    CHK(tm.make());
+    const hipEvent_t e0 = tm.e0, e1 = tm.e1;
     (void)hipEventRecord(e0, ds->stream);
     if (topk)
         rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_TOPK>(ds, tile, ta, ds->stream, false)
@@ -935,8 +1021,6 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
     const bool synced = hipEventSynchronize(e1) == hipSuccess;
     float k = 0;
     if (synced) (void)hipEventElapsedTime(&k, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     if (rc == EPISDET_OK && !synced) rc = EPISDET_ERR_HIP;
     *ms = k;
     return rc;
@@ -1295,9 +1379,13 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     if (o->candidate_cap > 0) cap = (unsigned long long)o->candidate_cap;
     else if (topk) cap = std::max<unsigned long long>(cap, std::min<unsigned long long>(1ull << 26, 32ull * topk * ds->cu));
     cap = std::max(cap, unit_sets_max(pl));
+    // a no-allocation call has the slots of `scratch` (at least one unit's)
+    if (t_scr) cap = std::min<unsigned long long>(cap, ds->cand_cap);
     CHK(ensure_cand(ds, (unsigned)cap));
     const ScoreCtx &sc = pl.sc;
-    if (order == 3 && !topk) {
+    // (the seeding pre-pass runs a top-k search with buffers of its own, so a
+    // no-allocation call goes without it: same winner, the bounds start later)
+    if (order == 3 && !topk && !t_scr) {
         unsigned key = 0;
         if (seed_from_pairs(ds, score, sc.margin, &key)) pl.seed_key = key;
     }
@@ -1556,6 +1644,7 @@ const char *episdet_status_string(int status)
     case EPISDET_ERR_ALLOC: return "allocation failed";
     case EPISDET_ERR_CANDIDATES: return "candidate buffer overflow";
     case EPISDET_ERR_GENOTYPES: return "a sample is marked in both genotype planes of a SNP";
+    case EPISDET_ERR_SCRATCH: return "the search needs more than the no-allocation entry point's scratch memory";
     default: return "internal error";
     }
 }
@@ -1704,6 +1793,132 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
     if (opt->path == EPISDET_PATH_PLAIN && opt->order == 3 && res->rounds_run < res->rounds && res->rounds > 0)
         res->sets = (long long)((double)res->sets * (double)res->rounds_run / (double)res->rounds);
     return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Per-device set-up of the no-allocation search: the device address of
+// `scratch` (each device holds its own copy at its own address) and two
+// timing events, made once with that device current.
+constexpr int SCRATCH_MAX_DEVICES = 64;
+struct ScratchDevice {
+    unsigned char *base = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+std::mutex g_scratch_mu;      // no-allocation calls of this process run one at a time
+ScratchDevice g_scratch_dev[SCRATCH_MAX_DEVICES];
+
+// Called with g_scratch_mu held and `dev` current.
+int scratch_device(int dev, ScratchDevice **out)
+{
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return EPISDET_ERR_INVALID;
+    ScratchDevice &sd = g_scratch_dev[dev];
+    if (!sd.base) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return EPISDET_ERR_INTERNAL;
+        if (!sd.ev[0]) HIPCHK(hipEventCreate(&sd.ev[0]));
+        if (!sd.ev[1]) HIPCHK(hipEventCreate(&sd.ev[1]));
+        sd.base = (unsigned char *)ptr;
+    }
+    *out = &sd;
+    return EPISDET_OK;
+}
+
+// Bytes of `scratch` a no-allocation search of this dataset needs, and the
+// regions it takes (device addresses from base).
+size_t scratch_layout(const episdet_dataset *ds, int order, int score, unsigned char *base, ScratchPlan &sp)
+{
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = off;
+        off = (off + bytes + 255) & ~(size_t)255;
+        return base + at;
+    };
+    sp.ctl = (Ctl *)take(sizeof(Ctl));
+    sp.jobargs = take(1024);
+    sp.jobstart = (unsigned long long *)take(2 * sizeof(unsigned long long));
+    sp.err = (unsigned long long *)take(sizeof(unsigned long long));
+    sp.cand = (unsigned long long *)take(sizeof(unsigned long long) * SCRATCH_CAND);
+    sp.cand_s32 = (float *)take(sizeof(float) * SCRATCH_CAND);
+    sp.cand_s64 = (double *)take(sizeof(double) * SCRATCH_CAND);
+    if (order == 3) {
+        // every 3-way unit enumeration uses Y and Z tiles of at least 32 SNPs
+        const size_t t = ((size_t)ds->n + 31) / 32;
+        sp.max_groups = t * (t + 1) / 2;
+        sp.gstart = (unsigned long long *)take(sizeof(unsigned long long) * (sp.max_groups + 1));
+        sp.groups = (int2 *)take(sizeof(int2) * sp.max_groups);
+    }
+    if (!ds->tab[score].ready) sp.t64 = (double *)take(sizeof(double) * ((size_t)ds->N[0] + ds->N[1] + 2));
+    if (order == 3 && !ds->pairs) sp.pairs = (uint4 *)take((size_t)ds->n * ds->n * 2 * sizeof(uint4));
+    return off;
+}
+
+}  // namespace
+
+extern "C" {
+
+int episdet_run_noalloc(episdet_dataset *ds, const episdet_options *opt, episdet_result *res)
+{
+    if (!ds || !opt || !res) return EPISDET_ERR_INVALID;
+    if (opt->order != 2 && opt->order != 3) return EPISDET_ERR_INVALID;
+    if (opt->score != EPISDET_SCORE_K2 && opt->score != EPISDET_SCORE_MI) return EPISDET_ERR_INVALID;
+    // the plain path, the rebuild and the tuner allocate buffers of their own
+    if (opt->path == EPISDET_PATH_PLAIN || opt->rebuild || opt->autotune) return EPISDET_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    HIPCHK(hipSetDevice(ds->dev));
+    ScratchDevice *sd = nullptr;
+    CHK(scratch_device(ds->dev, &sd));
+    ScratchPlan sp;
+    if (scratch_layout(ds, opt->order, opt->score, sd->base, sp) > SCRATCH_BYTES) {
+        std::fprintf(stderr,
+                     "episdet: episdet_run_noalloc: a %d-way search of %d SNPs x %d samples needs more than the "
+                     "%zu MiB of its static scratch memory\n",
+                     opt->order, ds->n, ds->N[0] + ds->N[1], SCRATCH_BYTES >> 20);
+        return EPISDET_ERR_SCRATCH;
+    }
+    sp.ev[0] = sd->ev[0];
+    sp.ev[1] = sd->ev[1];
+    // Point the dataset's per-search buffers at `scratch` for this call and
+    // restore them afterwards; the FP64 table and the pair table built in
+    // `scratch` are forgotten, since the next call overwrites them.
+    struct Bind {
+        episdet_dataset *ds;
+        int score;
+        Ctl *ctl;
+        void *jobargs;
+        unsigned long long *jobstart, *cand;
+        float *cand_s32;
+        double *cand_s64;
+        unsigned cand_cap;
+        uint4 *pairs;
+        ScoreTables tab;
+        ~Bind()
+        {
+            t_scr = nullptr;
+            ds->ctl = ctl;
+            ds->jobargs = jobargs;
+            ds->jobstart = jobstart;
+            ds->cand = cand;
+            ds->cand_s32 = cand_s32;
+            ds->cand_s64 = cand_s64;
+            ds->cand_cap = cand_cap;
+            ds->pairs = pairs;
+            ds->tab[score] = tab;
+        }
+    } bind{ds,           opt->score,    ds->ctl,      ds->jobargs, ds->jobstart, ds->cand,
+           ds->cand_s32, ds->cand_s64, ds->cand_cap, ds->pairs,   ds->tab[opt->score]};
+    ds->ctl = sp.ctl;
+    ds->jobargs = sp.jobargs;
+    ds->jobstart = sp.jobstart;
+    ds->cand = sp.cand;
+    ds->cand_s32 = sp.cand_s32;
+    ds->cand_s64 = sp.cand_s64;
+    ds->cand_cap = SCRATCH_CAND;
+    t_scr = &sp;
+    return episdet_run(ds, opt, res);
 }
 
 int episdet_autotune(episdet_dataset *ds, int order, int score, episdet_tune_entry *entries, int capacity,

@@ -29,7 +29,8 @@ MAF_UNIFORM, MAF_BETA = 0, 1
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "no gfx950 device", 3: "HIP runtime error",
           4: "allocation failed", 5: "candidate buffer overflow", 6: "internal error",
-          7: "a sample is marked in both genotype planes of a SNP"}
+          7: "a sample is marked in both genotype planes of a SNP",
+          8: "the search needs more than the no-allocation entry point's scratch memory"}
 
 
 class NativeMissing(RuntimeError):
@@ -96,7 +97,7 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 
 # Every symbol include/episdet.h and include/episdet_data.h declare.
 SEARCH_SYMBOLS = ("episdet_search", "episdet_dataset_create", "episdet_dataset_destroy", "episdet_run",
-                  "episdet_run_topk", "episdet_autotune", "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
+                  "episdet_run_noalloc", "episdet_run_topk", "episdet_autotune", "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
                   "episdet_checksum", "episdet_n_choose_k", "episdet_reference_rounds", "episdet_status_string",
                   "episdet_device_info")
 DATA_SYMBOLS = ("episdet_gen_defaults", "episdet_generate", "episdet_padded_words", "episdet_write_dataset",
@@ -141,6 +142,8 @@ def search_lib():
         lib.episdet_dataset_destroy.restype = None
         lib.episdet_run.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result)]
         lib.episdet_run.restype = C.c_int
+        lib.episdet_run_noalloc.argtypes = [C.c_void_p, C.POINTER(Options), C.POINTER(Result)]
+        lib.episdet_run_noalloc.restype = C.c_int
         lib.episdet_run_topk.argtypes = [C.c_void_p, C.POINTER(Options), C.c_int, C.POINTER(C.c_ulonglong),
                                          C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(Result)]
         lib.episdet_run_topk.restype = C.c_int
@@ -366,6 +369,16 @@ class Dataset:
         rc = self._lib.episdet_run(self._h, C.byref(o), C.byref(r))
         if rc:
             raise EpisdetError(rc, "episdet_run")
+        return r.as_dict()
+
+    def run_noalloc(self, order, score=K2, path=PATH_AUTO, tile=0, candidate_cap=0, topk=0) -> dict:
+        """episdet_run_noalloc: the same search with no device allocation
+        (per-search memory in the library's static scratch array)."""
+        o = Options(order=order, score=score, path=path, tile=tile, candidate_cap=candidate_cap, topk=topk)
+        r = Result()
+        rc = self._lib.episdet_run_noalloc(self._h, C.byref(o), C.byref(r))
+        if rc:
+            raise EpisdetError(rc, "episdet_run_noalloc")
         return r.as_dict()
 
     def autotune(self, order, score=K2):
