@@ -341,6 +341,13 @@ constexpr int M2_YROW = 2, M2_ZROW = 2 + 2 * M2_TY;
 // Reads at or past num_records (the class's plane bytes) return zero.
 typedef int i32x4_t __attribute__((ext_vector_type(4)));
 
+// LDS byte address of a pointer into a __shared__ array: the low 32 bits of
+// its generic address (the generic LDS aperture keeps the LDS offset there).
+// Taking them directly, rather than casting to an LDS pointer, spares the
+// null check of that cast, so the chunk loads compute their destinations in
+// 32-bit registers instead of keeping 64-bit pointers live.
+__device__ __forceinline__ unsigned lds_addr(const void *p) { return (unsigned)(size_t)p; }
+
 // One 16-byte-per-lane load from the buffer into LDS: lane L's data lands at
 // LDS byte address lds + 16 L.  Written as inline assembly on purpose: the
 // compiler cannot tell that these writes go to the other buffer of the
@@ -352,10 +359,9 @@ typedef int i32x4_t __attribute__((ext_vector_type(4)));
 // does not honour it in a clobber list); that is sound because no compiler-generated code in
 // these kernels uses m0, which tests/test_device_asm.py checks on the
 // assembly the Makefile's flags produce.
-__device__ __forceinline__ void lds_dma16(i32x4_t rsrc, const uint32_t *lds, unsigned voffset, unsigned soffset)
+__device__ __forceinline__ void lds_dma16(i32x4_t rsrc, unsigned lds, unsigned voffset, unsigned soffset)
 {
-    const unsigned m0 = (unsigned)__builtin_amdgcn_readfirstlane(
-        (int)(unsigned)(size_t)(const __attribute__((address_space(3))) uint32_t *)lds);
+    const unsigned m0 = (unsigned)__builtin_amdgcn_readfirstlane((int)lds);
     asm volatile("s_mov_b32 m0, %0\n\tbuffer_load_dwordx4 %1, %2, %3 offen lds"
                  :
                  : "s"(m0), "v"(voffset), "s"(rsrc), "s"(soffset)
@@ -396,12 +402,12 @@ __device__ __forceinline__ void m2_row_offsets(int W, int X, int y0, int z0, int
 }
 
 __device__ __forceinline__ void m2_issue_buf(i32x4_t rsrc, const unsigned (&roff)[M2_LOADS_PER_WAVE], int w0,
-                                             uint32_t *buf, int wih)
+                                             unsigned buf, int wih)
 {
 #pragma unroll
     for (int k = 0; k < M2_LOADS_PER_WAVE; k++) {
         const int ii = wih + 4 * k;
-        if (ii < M2_LOADS) lds_dma16(rsrc, buf + ii * 8 * M2_KW, roff[k], (unsigned)(w0 * 4));
+        if (ii < M2_LOADS) lds_dma16(rsrc, buf + (unsigned)(ii * 8 * M2_KW * 4), roff[k], (unsigned)(w0 * 4));
     }
 }
 
@@ -517,12 +523,12 @@ __global__ __launch_bounds__(512) void k_triplets_mfma2(BatchTri bt)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
 
-        if (nch_mine > 0) m2_issue_buf(rsrc, roff, 0, buf0, wy);
+        if (nch_mine > 0) m2_issue_buf(rsrc, roff, 0, lds_addr(buf0), wy);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int c = 0; c < nch; c++) {
             uint32_t *cur = (c & 1) ? buf1 : buf0;
-            if (c + 1 < nch_mine) m2_issue_buf(rsrc, roff, (c + 1) * M2_KW, (c & 1) ? buf0 : buf1, wy);
+            if (c + 1 < nch_mine) m2_issue_buf(rsrc, roff, (c + 1) * M2_KW, lds_addr((c & 1) ? buf0 : buf1), wy);
             if (c < nch_mine) {
 #pragma unroll
                 for (int s = 0; s < M2_KW / 8; s++) {

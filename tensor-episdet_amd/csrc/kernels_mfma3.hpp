@@ -54,12 +54,12 @@ __device__ __forceinline__ void m3_row_offsets(int W, int X, int y0, int z0, int
 }
 
 __device__ __forceinline__ void m3_issue_buf(i32x4_t rsrc, const unsigned (&roff)[M3_LOADS_PER_WAVE], int w0,
-                                             uint32_t *buf, int wih)
+                                             unsigned buf, int wih)
 {
 #pragma unroll
     for (int k = 0; k < M3_LOADS_PER_WAVE; k++) {
         const int ii = wih + 2 * k;
-        if (ii < M3_LOADS) lds_dma16(rsrc, buf + ii * M3_ROWS_PER_LOAD * M3_KW, roff[k], (unsigned)(w0 * 4));
+        if (ii < M3_LOADS) lds_dma16(rsrc, buf + (unsigned)(ii * M3_ROWS_PER_LOAD * M3_KW * 4), roff[k], (unsigned)(w0 * 4));
     }
 }
 
@@ -136,12 +136,12 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma3(BatchTri bt)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
 
-        if (nch_mine > 0) m3_issue_buf(rsrc, roff, 0, buf0, wy);
+        if (nch_mine > 0) m3_issue_buf(rsrc, roff, 0, lds_addr(buf0), wy);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int c = 0; c < nch; c++) {
             uint32_t *cur = (c & 1) ? buf1 : buf0;
-            if (c + 1 < nch_mine) m3_issue_buf(rsrc, roff, (c + 1) * M3_KW, (c & 1) ? buf0 : buf1, wy);
+            if (c + 1 < nch_mine) m3_issue_buf(rsrc, roff, (c + 1) * M3_KW, lds_addr((c & 1) ? buf0 : buf1), wy);
             if (c < nch_mine) {
 #pragma unroll
                 for (int s = 0; s < M3_KW / 8; s++) {

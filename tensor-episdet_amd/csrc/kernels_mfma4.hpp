@@ -79,13 +79,13 @@ __device__ __forceinline__ void m4_row_offsets(int W, int X0, int y0, int z0, in
 
 template <int KW>
 __device__ __forceinline__ void m4_issue_buf(i32x4_t rsrc, const unsigned (&roff)[M4Geo<KW>::LOADS_PER_WAVE], int w0,
-                                             uint32_t *buf, int wih)
+                                             unsigned buf, int wih)
 {
     using G = M4Geo<KW>;
 #pragma unroll
     for (int k = 0; k < G::LOADS_PER_WAVE; k++) {
         const int ii = wih + 2 * k;
-        if (ii < G::LOADS) lds_dma16(rsrc, buf + ii * G::ROWS_PER_LOAD * KW, roff[k], (unsigned)(w0 * 4));
+        if (ii < G::LOADS) lds_dma16(rsrc, buf + (unsigned)(ii * G::ROWS_PER_LOAD * KW * 4), roff[k], (unsigned)(w0 * 4));
     }
 }
 
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
         // wave is past chunk c - 1, whose buffer it reuses
 #pragma unroll
         for (int k = 0; k + 1 < G::NBUF; k++)
-            if (k < nch_mine) m4_issue_buf<KW>(rsrc, roff, k * KW, bufs + k * G::BUF_WORDS, xs);
+            if (k < nch_mine) m4_issue_buf<KW>(rsrc, roff, k * KW, lds_addr(bufs + k * G::BUF_WORDS), xs);
         for (int c = 0; c < nch; c++) {
             // this wave's loads of chunk c are done once at most those of the
             // chunks after it that are already issued (NBUF - 2 at most) are
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
             __syncthreads();
             if (c + G::NBUF - 1 < nch_mine)
                 m4_issue_buf<KW>(rsrc, roff, (c + G::NBUF - 1) * KW,
-                                 bufs + ((c + G::NBUF - 1) % G::NBUF) * G::BUF_WORDS, xs);
+                                 lds_addr(bufs + ((c + G::NBUF - 1) % G::NBUF) * G::BUF_WORDS), xs);
             if (c < nch_mine) {
                 const uint32_t *cur = bufs + (c % G::NBUF) * G::BUF_WORDS;
 #pragma unroll

@@ -58,12 +58,12 @@ __device__ __forceinline__ void m5_row_offsets(int W, int X0, int y0, int z0, in
 }
 
 __device__ __forceinline__ void m5_issue_buf(i32x4_t rsrc, const unsigned (&roff)[M5_LOADS_PER_WAVE], int w0,
-                                             uint32_t *buf, int w)
+                                             unsigned buf, int w)
 {
 #pragma unroll
     for (int k = 0; k < M5_LOADS_PER_WAVE; k++) {
         const int ii = w + 4 * k;
-        if (ii < M5_LOADS) lds_dma16(rsrc, buf + ii * M5_ROWS_PER_LOAD * M5_KW, roff[k], (unsigned)(w0 * 4));
+        if (ii < M5_LOADS) lds_dma16(rsrc, buf + (unsigned)(ii * M5_ROWS_PER_LOAD * M5_KW * 4), roff[k], (unsigned)(w0 * 4));
     }
 }
 
@@ -164,8 +164,8 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma5(BatchTri bt)
         for (int k = 0; k < M5_NBUF - 1; k++) {
             if (k < nch) {
                 uint32_t *nb = smem + k * M5_BUF_WORDS;
-                if (k < nch0) m5_issue_buf(rs0, roff0, k * M5_KW, nb, w);
-                else m5_issue_buf(rs1, roff1, (k - nch0) * M5_KW, nb, w);
+                if (k < nch0) m5_issue_buf(rs0, roff0, k * M5_KW, lds_addr(nb), w);
+                else m5_issue_buf(rs1, roff1, (k - nch0) * M5_KW, lds_addr(nb), w);
             }
         }
         // two loops, one per class, so each contracts into one fixed set of
@@ -185,8 +185,8 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma5(BatchTri bt)
                 const int tn = t + M5_NBUF - 1;
                 if (tn < nch) {
                     uint32_t *nb = smem + (tn % M5_NBUF) * M5_BUF_WORDS;
-                    if (tn < nch0) m5_issue_buf(rs0, roff0, tn * M5_KW, nb, w);
-                    else m5_issue_buf(rs1, roff1, (tn - nch0) * M5_KW, nb, w);
+                    if (tn < nch0) m5_issue_buf(rs0, roff0, tn * M5_KW, lds_addr(nb), w);
+                    else m5_issue_buf(rs1, roff1, (tn - nch0) * M5_KW, lds_addr(nb), w);
                 }
                 m5_chunk(smem + (t % M5_NBUF) * M5_BUF_WORDS, xr, yr, zr0, zr1, h, acc[c], bt.sh);
             }

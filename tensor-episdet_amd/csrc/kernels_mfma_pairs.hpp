@@ -86,12 +86,12 @@ __device__ __forceinline__ void p2_row_offsets(int W, int a0, int b0, int wih, i
 }
 
 __device__ __forceinline__ void p2_issue_buf(i32x4_t rsrc, const unsigned (&roff)[P2_LOADS_PER_WAVE], int w0,
-                                             uint32_t *buf, int wih)
+                                             unsigned buf, int wih)
 {
 #pragma unroll
     for (int k = 0; k < P2_LOADS_PER_WAVE; k++) {
         const int ii = wih + 4 * k;
-        lds_dma16(rsrc, buf + ii * P2_ROWS_PER_LOAD * P2_KW, roff[k], (unsigned)(w0 * 4));
+        lds_dma16(rsrc, buf + (unsigned)(ii * P2_ROWS_PER_LOAD * P2_KW * 4), roff[k], (unsigned)(w0 * 4));
     }
 }
 
@@ -159,12 +159,12 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
 #pragma unroll
                 for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
 
-        if (nch_mine > 0) p2_issue_buf(rsrc, roff, 0, buf0, wy);
+        if (nch_mine > 0) p2_issue_buf(rsrc, roff, 0, lds_addr(buf0), wy);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         for (int c = 0; c < nch; c++) {
             uint32_t *cur = (c & 1) ? buf1 : buf0;
-            if (c + 1 < nch_mine) p2_issue_buf(rsrc, roff, (c + 1) * P2_KW, (c & 1) ? buf0 : buf1, wy);
+            if (c + 1 < nch_mine) p2_issue_buf(rsrc, roff, (c + 1) * P2_KW, lds_addr((c & 1) ? buf0 : buf1), wy);
             if (c < nch_mine) {
 #pragma unroll
                 for (int s = 0; s < P2_KW / 8; s++) {

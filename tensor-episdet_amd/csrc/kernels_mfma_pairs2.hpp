@@ -58,11 +58,11 @@ __device__ __forceinline__ void p3_row_offsets(int W, int a0, int b0, int w, int
 }
 
 __device__ __forceinline__ void p3_issue_buf(i32x4_t rsrc, const unsigned (&roff)[P3_LOADS_PER_WAVE], int w0,
-                                             uint32_t *buf, int w)
+                                             unsigned buf, int w)
 {
 #pragma unroll
     for (int k = 0; k < P3_LOADS_PER_WAVE; k++)
-        lds_dma16(rsrc, buf + (w + 4 * k) * P3_ROWS_PER_LOAD * P3_KW, roff[k], (unsigned)(w0 * 4));
+        lds_dma16(rsrc, buf + (unsigned)((w + 4 * k) * P3_ROWS_PER_LOAD * P3_KW * 4), roff[k], (unsigned)(w0 * 4));
 }
 
 template <int C>
@@ -144,8 +144,8 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma2(BatchPair bp)
 
         // chunk t of the sequence: cases chunk t, or controls chunk t - nch0
         if (nch > 0) {
-            if (nch0 > 0) p3_issue_buf(rs0, roff0, 0, smem, w);
-            else p3_issue_buf(rs1, roff1, 0, smem, w);
+            if (nch0 > 0) p3_issue_buf(rs0, roff0, 0, lds_addr(smem), w);
+            else p3_issue_buf(rs1, roff1, 0, lds_addr(smem), w);
         }
         // two loops, one per class, so each contracts into one fixed set of
         // accumulators
@@ -158,8 +158,8 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma2(BatchPair bp)
                 const int tn = t + 1;
                 if (tn < nch) {
                     uint32_t *nb = smem + (tn & 1) * P3_BUF_WORDS;
-                    if (tn < nch0) p3_issue_buf(rs0, roff0, tn * P3_KW, nb, w);
-                    else p3_issue_buf(rs1, roff1, (tn - nch0) * P3_KW, nb, w);
+                    if (tn < nch0) p3_issue_buf(rs0, roff0, tn * P3_KW, lds_addr(nb), w);
+                    else p3_issue_buf(rs1, roff1, (tn - nch0) * P3_KW, lds_addr(nb), w);
                 }
                 p3_chunk(smem + (t & 1) * P3_BUF_WORDS, ar0, ar1, br0, br1, h, acc[c]);
             }
