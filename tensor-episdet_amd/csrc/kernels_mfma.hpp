@@ -52,25 +52,30 @@ struct MfmaStage {
 };
 
 // Global loads of chunk w0 for one class, by the 256 threads t of that half.
+// Rows are W words, a multiple of half a chunk: words at or past W (the second
+// half of a last chunk) read as zero, which counts nothing.
 __device__ __forceinline__ void mf_load(const uint32_t *__restrict__ base, int W, int X, int y0, int z0, int w0,
                                         int t, MfmaStage &st)
 {
     const int w = t % MF_KW;                 // every (y, w) position of this thread has the same w
+    const bool in = w0 + w < W;
     const uint32_t *xr = base + (size_t)X * 2 * W + w0 + w;
-    st.x0 = xr[0];
-    st.x1 = xr[W];
+    st.x0 = in ? xr[0] : 0u;
+    st.x1 = in ? xr[W] : 0u;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int y = t / MF_KW + 4 * q;     // MF_TY * MF_KW = 2048 positions
         const uint32_t *yr = base + (size_t)(y0 + y) * 2 * W + w0 + w;
-        st.y0v[q] = yr[0];
-        st.y1v[q] = yr[W];
+        st.y0v[q] = in ? yr[0] : 0u;
+        st.y1v[q] = in ? yr[W] : 0u;
     }
 #pragma unroll
     for (int q = 0; q < 8; q++) {
         const int e = t + 256 * q;           // 2 * MF_TZ * MF_KW / 4 = 2048 uint4
         const int row = e / (MF_KW / 4), c4 = e % (MF_KW / 4);
-        st.z[q] = *(const uint4 *)(base + ((size_t)(z0 + (row >> 1)) * 2 + (row & 1)) * W + w0 + c4 * 4);
+        st.z[q] = w0 + c4 * 4 < W
+                      ? *(const uint4 *)(base + ((size_t)(z0 + (row >> 1)) * 2 + (row & 1)) * W + w0 + c4 * 4)
+                      : make_uint4(0u, 0u, 0u, 0u);
     }
 }
 
@@ -185,8 +190,9 @@ __global__ __launch_bounds__(512) void k_triplets_mfma(BatchTri bt)
         const unsigned long long u = uu - bt.job_start[j];
         const uint32_t *base = cls ? p.p1 : p.p0;
         const int W = cls ? p.W1 : p.W0;
-        const int nch_mine = W / MF_KW;
-        const int nch = (p.W0 > p.W1 ? p.W0 : p.W1) / MF_KW;
+        // a last chunk may hold half its words (W is a multiple of MF_KW / 2)
+        const int nch_mine = (W + MF_KW - 1) / MF_KW;
+        const int nch = ((p.W0 > p.W1 ? p.W0 : p.W1) + MF_KW - 1) / MF_KW;
         g = next_group(p.gstart, p.ngroups, u, g);
         const int X = (int)(u - p.gstart[g]);
         const int2 gt = p.groups[g];

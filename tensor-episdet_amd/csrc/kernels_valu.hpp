@@ -897,6 +897,17 @@ __global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)
     if ((int)blockIdx.x < p.count) rescore_one(p, blockIdx.x);
 }
 
+// FP64 re-scoring of the candidates one search appended, with their count
+// read on the device (cand[0]; nothing when the overflow flag cand[1] is set
+// or the count exceeds cap: the host then searches again in windows), so it
+// follows the search on the stream with no host round trip.
+__global__ __launch_bounds__(64) void k_rescore_after(RescoreArgs p, const unsigned *cand, unsigned cap)
+{
+    const unsigned count = cand[0];
+    if (cand[1] || count > cap) return;
+    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) rescore_one(p, (int)i);
+}
+
 // FP64 re-scoring of the candidates of many searches; the candidate counts
 // are read on the device, so the whole batch needs no host round trip.
 struct RescoreJob {
@@ -1007,12 +1018,14 @@ __global__ void k_term_error(int score, int N, float invN, float k0, float c32, 
 }
 
 // Small fill kernels (no hipMemsetAsync, so the sequence stays capturable).
+// queue0: first work unit; best0: starting global bound (key), 0 for +inf.
 __global__ void k_init_search(unsigned long long *queue, unsigned *best_key, unsigned *cand_count,
-                              unsigned long long *best_key64, unsigned long long *checksum)
+                              unsigned long long *best_key64, unsigned long long *checksum,
+                              unsigned long long queue0, unsigned best0)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        *queue = 0ull;
-        *best_key = 0xFF800000u;                 // key of +inf
+        *queue = queue0;
+        *best_key = best0 ? best0 : 0xFF800000u; // key of +inf
         cand_count[0] = 0u;
         cand_count[1] = 0u;                      // overflow flag
         *best_key64 = 0xFFF0000000000000ull;     // key of +inf (double)

@@ -36,8 +36,18 @@ namespace {
 
 constexpr int BLOCK = 256;            // the reference's BLOCK_SIZE (Makefile:2)
 constexpr int PLAIN_GROUP = 16;       // rounds per FP4 count launch of the plain path
+// Words per plane row on the device: the samples rounded up to 1024 (the
+// reference's own padding), a whole chunk of every kernel that stages
+// through LDS DMA (16 or 32 words) and half a chunk of MFMA version 1, which
+// reads the missing half of its last chunk as zeros.
+constexpr int W_ALIGN = 32;
+static_assert(M2_KW % 16 == 0 && 32 % M2_KW == 0 && MF_KW == 2 * W_ALIGN, "chunk widths divide the row alignment");
 constexpr int NUM_STREAMS = 8;        // the reference's NUM_STREAMS (Makefile:2)
 constexpr unsigned DEFAULT_CAP = 1u << 20;
+// Candidates copied back with the control block right after a search (a
+// search that appends more fetches the rest in a second copy).
+constexpr unsigned STAGE_CAND = 1024;
+constexpr size_t STAGE_BYTES = 256 + (size_t)STAGE_CAND * (sizeof(unsigned long long) + sizeof(float) + sizeof(double));
 
 #define HIPCHK(expr)                                                                                    \
     do {                                                                                                \
@@ -173,6 +183,9 @@ struct episdet_dataset {
     int cu = 0;
     void *jobargs = nullptr;                  // device copy of one kernel argument block
     unsigned long long *jobstart = nullptr;   // {0, units} for a batch of one
+    std::vector<unsigned char> args_sent;     // what jobargs / jobstart last received (upload_single)
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // search start / end, re-score end
+    unsigned char *stage = nullptr;           // pinned: control block + the first candidates (STAGE_CAND)
     std::vector<unsigned long long> topk_packed;   // last top-k result, best first
     std::vector<double> topk_scores;
 };
@@ -507,13 +520,24 @@ void launch_tri_mfma(int tile, int blocks, const BatchTri &bt, hipStream_t s)
 static_assert(sizeof(TriArgs) <= 1024 && sizeof(PairArgs) <= 1024, "argument slot too small");
 
 // Copies one argument block to the dataset's device slot, for a batch of one.
+// The kernels only read the slot, so a block identical to the last one sent
+// is not sent again (repeated searches of one dataset skip the copies and
+// their synchronisation).  Inside a no-allocation call the slot is in the
+// shared scratch array, which other datasets' calls overwrite: always sent.
 template <typename A>
 int upload_single(episdet_dataset *ds, const A &a, unsigned long long units)
 {
     const unsigned long long js[2] = {0ull, units};
+    unsigned char key[sizeof(A) + sizeof js];
+    std::memcpy(key, &a, sizeof(A));
+    std::memcpy(key + sizeof(A), js, sizeof js);
+    if (!t_scr && ds->args_sent.size() == sizeof key && std::memcmp(ds->args_sent.data(), key, sizeof key) == 0)
+        return EPISDET_OK;
+    ds->args_sent.clear();
     HIPCHK(hipMemcpyAsync(ds->jobargs, &a, sizeof(A), hipMemcpyHostToDevice, ds->stream));
     HIPCHK(hipMemcpyAsync(ds->jobstart, js, sizeof js, hipMemcpyHostToDevice, ds->stream));
     HIPCHK(hipStreamSynchronize(ds->stream));
+    if (!t_scr) ds->args_sent.assign(key, key + sizeof key);
     return EPISDET_OK;
 }
 
@@ -701,10 +725,13 @@ void fill_pairs(episdet_dataset *ds, PairArgs &a, bool mfma, int tile)
     }
 }
 
-int init_ctl(episdet_dataset *ds)
+// Resets the dataset's control block on its stream: queue at u0, global
+// bound at seed_key (0: +inf), no candidates.  Both values travel as kernel
+// arguments, so no host memory has to outlive the call.
+int init_ctl(episdet_dataset *ds, unsigned long long u0 = 0, unsigned seed_key = 0)
 {
     hipLaunchKernelGGL(k_init_search, dim3(1), dim3(64), 0, ds->stream, &ds->ctl->queue, &ds->ctl->best32,
-                       &ds->ctl->cand_count, &ds->ctl->best64, &ds->ctl->checksum);
+                       &ds->ctl->cand_count, &ds->ctl->best64, &ds->ctl->checksum, u0, seed_key);
     HIPCHK(hipGetLastError());
     return EPISDET_OK;
 }
@@ -755,13 +782,9 @@ void unpack(int order, unsigned long long p, int idx[3])
     }
 }
 
-// FP64 re-scoring of `count` packed sets already on the device in ds->cand.
-int rescore(episdet_dataset *ds, int order, int score, int count, double *ms)
+// Arguments of the FP64 re-scoring of the sets in ds->cand (count unset).
+RescoreArgs rescore_args(const episdet_dataset *ds, int order, int score)
 {
-    if (count <= 0) {
-        *ms = 0;
-        return EPISDET_OK;
-    }
     RescoreArgs r;
     std::memset(&r, 0, sizeof r);
     r.p0 = ds->planes[0];
@@ -777,8 +800,19 @@ int rescore(episdet_dataset *ds, int order, int score, int count, double *ms)
     r.pairs = ds->pairs;
     r.t64 = ds->tab[score].t64;
     r.cand = ds->cand;
-    r.count = count;
     r.out_s64 = ds->cand_s64;
+    return r;
+}
+
+// FP64 re-scoring of `count` packed sets already on the device in ds->cand.
+int rescore(episdet_dataset *ds, int order, int score, int count, double *ms)
+{
+    if (count <= 0) {
+        *ms = 0;
+        return EPISDET_OK;
+    }
+    RescoreArgs r = rescore_args(ds, order, score);
+    r.count = count;
     Timer tm;
     CHK(tm.make());
     HIPCHK(hipEventRecord(tm.e0, ds->stream));
@@ -806,18 +840,11 @@ struct Collected {
     std::vector<std::pair<double, unsigned long long>> top;   // better() order, at most topk
 };
 
-// Adds the `count` candidates re-scored on the device (ds->cand,
-// ds->cand_s32, ds->cand_s64) to acc.
-int collect(episdet_dataset *ds, int count, Collected &acc)
+// Adds `count` re-scored candidates (host copies) to acc.
+int collect_host(const unsigned long long *hp, const float *h32, const double *h64, int count, Collected &acc)
 {
     if (count <= 0) return EPISDET_OK;
     const int score = acc.score;
-    std::vector<unsigned long long> hp(count);
-    std::vector<float> h32(count);
-    std::vector<double> h64(count);
-    HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h32.data(), ds->cand_s32, sizeof(float) * count, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
     for (int i = 0; i < count; i++) {
         if (std::isnan(h64[i])) return EPISDET_ERR_INTERNAL;   // inconsistent count tables
         const double s32 = score == EPISDET_SCORE_K2 ? (double)h32[i] : -(double)h32[i];
@@ -844,6 +871,27 @@ int collect(episdet_dataset *ds, int count, Collected &acc)
     acc.count += count;
     return EPISDET_OK;
 }
+
+// Adds the `count` candidates re-scored on the device (ds->cand,
+// ds->cand_s32, ds->cand_s64) to acc.
+int collect(episdet_dataset *ds, int count, Collected &acc)
+{
+    if (count <= 0) return EPISDET_OK;
+    std::vector<unsigned long long> hp(count);
+    std::vector<float> h32(count);
+    std::vector<double> h64(count);
+    HIPCHK(hipMemcpy(hp.data(), ds->cand, sizeof(unsigned long long) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h32.data(), ds->cand_s32, sizeof(float) * count, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(h64.data(), ds->cand_s64, sizeof(double) * count, hipMemcpyDeviceToHost));
+    return collect_host(hp.data(), h32.data(), h64.data(), count, acc);
+}
+
+// Host addresses of the parts of a dataset's pinned staging area.
+Ctl *stage_ctl(episdet_dataset *ds) { return (Ctl *)ds->stage; }
+unsigned long long *stage_packed(episdet_dataset *ds) { return (unsigned long long *)(ds->stage + 256); }
+float *stage_s32(episdet_dataset *ds) { return (float *)(ds->stage + 256 + 8 * (size_t)STAGE_CAND); }
+double *stage_s64(episdet_dataset *ds) { return (double *)(ds->stage + 256 + 12 * (size_t)STAGE_CAND); }
+static_assert(sizeof(Ctl) <= 256, "control block fits the staging header");
 
 // Writes the gathered winner (and, for top-k, the ranking) into res and ds.
 int finish(episdet_dataset *ds, int order, const Collected &acc, episdet_result *res)
@@ -882,9 +930,9 @@ int collect_best(episdet_dataset *ds, int order, int score, int count, episdet_r
 struct FusedPlan {
     int order = 0, score = 0, path = 0, tile = 0;
     bool mfma = false;
-    TriArgs ta;
-    PairArgs pa;
-    ScoreCtx sc;
+    TriArgs ta{};
+    PairArgs pa{};
+    ScoreCtx sc{};
     std::vector<int2> groups;
     std::vector<unsigned long long> gstart;
     unsigned long long *d_gstart = nullptr;
@@ -972,13 +1020,13 @@ unsigned long long unit_sets_max(const FusedPlan &pl)
     return 256ull * PAIR_TILES[pl.tile] * PAIR_TILES[pl.tile];
 }
 
-// Launches the plan over work units [u0, u1) (the whole search is
+// Issues the plan over work units [u0, u1) (the whole search is
 // [0, nunits)) with `cap` candidate slots and the global bound starting at
-// seed_key (0: +inf), and returns the kernel time.  The argument block is
-// uploaded before the first timing event, so the time covers the search
-// kernel alone.
-int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long long u0, unsigned long long u1,
-                 unsigned cap, unsigned seed_key, float *ms)
+// seed_key (0: +inf), between events e0 and e1 on the dataset's stream; does
+// not wait for it.  The argument block is uploaded before e0, so the events
+// time the search kernel alone.
+int issue_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long long u0, unsigned long long u1,
+                unsigned cap, unsigned seed_key, hipEvent_t e0, hipEvent_t e1)
 {
     if (cap > ds->cand_cap) return EPISDET_ERR_INTERNAL;
     TriArgs &ta = pl.ta;
@@ -1001,14 +1049,8 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
     // VALU 3-way kernel takes it by value)
     if (order == 3 && mfma) CHK(upload_single(ds, ta, ta.nunits));
     else if (order == 2) CHK(upload_single(ds, pa, pa.nunits));
-    CHK(init_ctl(ds));
-    if (u0) HIPCHK(hipMemcpyAsync(&ds->ctl->queue, &u0, sizeof u0, hipMemcpyHostToDevice, ds->stream));
-    if (seed_key)
-        HIPCHK(hipMemcpyAsync(&ds->ctl->best32, &seed_key, sizeof seed_key, hipMemcpyHostToDevice, ds->stream));
-    Timer tm;
-    CHK(tm.make());
-    const hipEvent_t e0 = tm.e0, e1 = tm.e1;
-    (void)hipEventRecord(e0, ds->stream);
+    CHK(init_ctl(ds, u0, seed_key));
+    HIPCHK(hipEventRecord(e0, ds->stream));
     if (topk)
         rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_TOPK>(ds, tile, ta, ds->stream, false)
                                 : dispatch_tri<MODE_TOPK>(ds, tile, ta, 0, ds->stream))
@@ -1017,13 +1059,56 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
         rc = order == 3 ? (mfma ? dispatch_tri_mfma<MODE_SEARCH>(ds, tile, ta, ds->stream, false)
                                 : dispatch_tri<MODE_SEARCH>(ds, tile, ta, 0, ds->stream))
                         : dispatch_pairs<MODE_SEARCH>(ds, mfma, tile, pa, ds->stream, false);
-    (void)hipEventRecord(e1, ds->stream);
-    const bool synced = hipEventSynchronize(e1) == hipSuccess;
+    HIPCHK(hipEventRecord(e1, ds->stream));
+    return rc;
+}
+
+// issue_fused, waiting for the search: its kernel time in *ms.
+int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long long u0, unsigned long long u1,
+                 unsigned cap, unsigned seed_key, float *ms)
+{
+    Timer tm;
+    CHK(tm.make());
+    int rc = issue_fused(ds, pl, topk, u0, u1, cap, seed_key, tm.e0, tm.e1);
+    const bool synced = hipEventSynchronize(tm.e1) == hipSuccess;
     float k = 0;
-    if (synced) (void)hipEventElapsedTime(&k, e0, e1);
+    if (synced) (void)hipEventElapsedTime(&k, tm.e0, tm.e1);
     if (rc == EPISDET_OK && !synced) rc = EPISDET_ERR_HIP;
     *ms = k;
     return rc;
+}
+
+// The whole search in one synchronisation: the launch, then on the same
+// stream the FP64 re-scoring of the candidates it appended (count read on the
+// device) and the copy of the control block and of the first STAGE_CAND
+// candidates to the dataset's pinned staging area.  *hc receives the control
+// block; the search's and the re-scoring's device times go to *kms, *rms.
+int launch_fused_rescored(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned cap, unsigned seed_key, float *kms,
+                          float *rms, Ctl *hc)
+{
+    int rc = issue_fused(ds, pl, topk, 0, pl.nunits, cap, seed_key, ds->ev[0], ds->ev[1]);
+    if (rc == EPISDET_OK) {
+        const RescoreArgs r = rescore_args(ds, pl.order, pl.score);
+        const unsigned grid = std::min<unsigned>(cap, 16u * (unsigned)ds->cu);
+        hipLaunchKernelGGL(k_rescore_after, dim3(grid), dim3(64), 0, ds->stream, r, &ds->ctl->cand_count, cap);
+        if (hipGetLastError() != hipSuccess) rc = EPISDET_ERR_HIP;
+    }
+    if (rc == EPISDET_OK && hipEventRecord(ds->ev[2], ds->stream) != hipSuccess) rc = EPISDET_ERR_HIP;
+    const size_t k = std::min<size_t>(cap, STAGE_CAND);
+    if (rc == EPISDET_OK &&
+        (hipMemcpyAsync(stage_ctl(ds), ds->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ds->stream) != hipSuccess ||
+         hipMemcpyAsync(stage_packed(ds), ds->cand, 8 * k, hipMemcpyDeviceToHost, ds->stream) != hipSuccess ||
+         hipMemcpyAsync(stage_s32(ds), ds->cand_s32, 4 * k, hipMemcpyDeviceToHost, ds->stream) != hipSuccess ||
+         hipMemcpyAsync(stage_s64(ds), ds->cand_s64, 8 * k, hipMemcpyDeviceToHost, ds->stream) != hipSuccess))
+        rc = EPISDET_ERR_HIP;
+    // wait even after an error, so no copy is left writing the staging area
+    const bool synced = hipStreamSynchronize(ds->stream) == hipSuccess;
+    if (rc != EPISDET_OK) return rc;
+    if (!synced) return EPISDET_ERR_HIP;
+    HIPCHK(hipEventElapsedTime(kms, ds->ev[0], ds->ev[1]));
+    HIPCHK(hipEventElapsedTime(rms, ds->ev[1], ds->ev[2]));
+    std::memcpy(hc, stage_ctl(ds), sizeof(Ctl));
+    return EPISDET_OK;
 }
 
 int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune);
@@ -1394,14 +1479,17 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     Collected acc;
     acc.score = score;
     acc.topk = topk;
-    float kms = 0;
-    CHK(launch_fused(ds, pl, topk, 0, pl.nunits, (unsigned)cap, pl.seed_key, &kms));
-    double kms_total = kms, rms_total = 0;
+    float kms = 0, rms = 0;
     Ctl hc;
-    HIPCHK(hipMemcpy(&hc, ds->ctl, sizeof hc, hipMemcpyDeviceToHost));
+    CHK(launch_fused_rescored(ds, pl, topk, (unsigned)cap, pl.seed_key, &kms, &rms, &hc));
+    double kms_total = kms, rms_total = 0;
     if (!hc.cand_over && hc.cand_count <= cap) {
-        CHK(rescore(ds, order, score, (int)hc.cand_count, &rms_total));
-        CHK(collect(ds, (int)hc.cand_count, acc));
+        // re-scored on the device behind the search; the first STAGE_CAND
+        // candidates are already on the host
+        rms_total = rms;
+        const int count = (int)hc.cand_count;
+        if (count <= (int)STAGE_CAND) CHK(collect_host(stage_packed(ds), stage_s32(ds), stage_s64(ds), count, acc));
+        else CHK(collect(ds, count, acc));
     } else {
         // more sets within the FP32 margin than slots: search again in windows
         CHK(run_sliced(ds, pl, topk, (unsigned)cap, hc, acc, &kms_total, &rms_total));
@@ -1709,9 +1797,15 @@ int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int 
     if (rc == EPISDET_OK && hipMalloc(&ds->jobargs, 1024) != hipSuccess) rc = EPISDET_ERR_ALLOC;
     if (rc == EPISDET_OK && hipMalloc(&ds->jobstart, 2 * sizeof(unsigned long long)) != hipSuccess)
         rc = EPISDET_ERR_ALLOC;
+    for (int e = 0; e < 3 && rc == EPISDET_OK; e++)
+        if (hipEventCreate(&ds->ev[e]) != hipSuccess) rc = EPISDET_ERR_HIP;
+    if (rc == EPISDET_OK && hipHostMalloc((void **)&ds->stage, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+        ds->stage = nullptr;
+        rc = EPISDET_ERR_ALLOC;
+    }
     for (int c = 0; c < 2 && rc == EPISDET_OK; c++) {
         const int Wv = (ds->N[c] + 31) / 32;
-        ds->W[c] = ((Wv + MF_KW - 1) / MF_KW) * MF_KW;   // a multiple of every kernel's chunk
+        ds->W[c] = ((Wv + W_ALIGN - 1) / W_ALIGN) * W_ALIGN;
         const size_t in_bytes = sizeof(uint32_t) * (size_t)numSNPs * 2 * in_w[c];
         const size_t dev_words = (size_t)ds->nrows * 2 * ds->W[c];
         uint32_t *stage = nullptr;
@@ -1757,6 +1851,9 @@ void episdet_dataset_destroy(episdet_dataset *ds)
     if (ds->cand) (void)hipFree(ds->cand);
     if (ds->cand_s32) (void)hipFree(ds->cand_s32);
     if (ds->cand_s64) (void)hipFree(ds->cand_s64);
+    for (hipEvent_t e : ds->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ds->stage) (void)hipHostFree(ds->stage);
     if (ds->stream) (void)hipStreamDestroy(ds->stream);
     delete ds;
 }
