@@ -283,7 +283,7 @@ def main():
     planted_ok = (tuple(r["idx"][:order]) in syn.planted) if syn.planted else None
 
     plain = None
-    if not args.no_plain and order == 3:
+    if not args.no_plain:
         # the same round loop with each of its two count kernels: FP4 MFMA
         # (the fused path's counting engine, so fused / plain is the gain of
         # fusion alone) and AND + v_bcnt on the VALU

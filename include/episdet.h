@@ -108,9 +108,10 @@ typedef struct {
                              (0: 128 x 64 units; 1: 64 x 64, classes in
                              sequence), FUSED_VALU 3-way 0-3, FUSED_VALU 2-way
                              0-2; PLAIN: the count kernel of its rounds, 0 =
-                             AND + v_bcnt on the VALU, 1 = 3-way only, FP4 MFMA
-                             (the 64 x 64 kernel of FUSED_MFMA tile 0 writing
-                             its counts; 16 rounds per launch)                   */
+                             AND + v_bcnt on the VALU, 1 = FP4 MFMA (3-way: the
+                             64 x 64 kernel of FUSED_MFMA tile 0 writing its
+                             counts, 16 rounds per launch; 2-way: the 128 x 64
+                             kernel of 2-way tile 0, 32 rounds per launch)       */
     int plain_rounds;     /* PLAIN only: stop after this many rounds (0 = all);
                              used to time a contiguous slice of the round loop   */
     int candidate_cap;    /* FP32 candidates kept per launch for FP64 re-scoring;

@@ -132,17 +132,28 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
     for (int cb = 0; cb < 4; cb++) br[cb] = P2_AROWS + 32 * cb + i;
 
     int j = 0;
-    for (;;) {
-        if (tid == 0) s_u = atomicAdd(bp.queue, 1ull);
-        __syncthreads();
-        const unsigned long long uu = s_u;
-        if (uu >= bp.total) break;
-        while (j + 1 < bp.njobs && bp.job_start[j + 1] <= uu) j++;
+    for (int iter = 0;; iter++) {
+        int a0, b0;
+        if constexpr (MODE == MODE_COUNTS) {
+            // one 128 x 64 tile of round blockIdx.x / 8, no queue
+            if (iter > 0) break;
+            const int r = (int)blockIdx.x >> 3, tl = (int)blockIdx.x & 7;
+            if (r >= bp.rounds) break;
+            a0 = bp.ab * 256 + (tl >> 2) * P2_TA;
+            b0 = (bp.bb0 + r) * 256 + (tl & 3) * P2_TB;
+        } else {
+            if (tid == 0) s_u = atomicAdd(bp.queue, 1ull);
+            __syncthreads();
+            const unsigned long long uu = s_u;
+            if (uu >= bp.total) break;
+            while (j + 1 < bp.njobs && bp.job_start[j + 1] <= uu) j++;
+            const unsigned long long u = uu - bp.job_start[j];
+            int tA, tB;
+            p2_unrank(u, tA, tB);
+            a0 = tA * P2_TA;
+            b0 = tB * P2_TB;
+        }
         const PairArgs &p = bp.jobs[j];
-        const unsigned long long u = uu - bp.job_start[j];
-        int tA, tB;
-        p2_unrank(u, tA, tB);
-        const int a0 = tA * P2_TA, b0 = tB * P2_TB;
         const uint32_t *base = cls ? p.p1 : p.p0;
         const int W = cls ? p.W1 : p.W0;
         const int nch_mine = W / P2_KW;
@@ -211,7 +222,8 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
                 const int A = a0 + al, B = b0 + bl;
                 bad[m] = INFINITY;
                 packed[m] = (unsigned long long)A | ((unsigned long long)B << 32);
-                if (!(A < B && B < p.n)) continue;
+                // the plain path's objective kernel checks the pair itself
+                if (MODE != MODE_COUNTS && !(A < B && B < p.n)) continue;
                 // row 2*al + pi lives in wave al/32, row tile (al/16)%2, tile
                 // row rho = 2*(al%16) + pi; column 2*bl' + pj in column tile
                 // bl/16, tile column 2*(bl%16) + pj.  Output register
@@ -235,6 +247,11 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
                     uint4 *o = p.pairs + ((size_t)A * p.n + B) * 2;
                     o[0] = make_uint4(n4c[0], n4c[1], n4c[2], n4c[3]);
                     o[1] = make_uint4(n4t[0], n4t[1], n4t[2], n4t[3]);
+                } else if constexpr (MODE == MODE_COUNTS) {
+                    const int ar = A - bp.ab * 256, bbr = B / 256 - bp.bb0, br = B % 256;
+                    uint32_t *rc = bp.counts + (size_t)bbr * bp.slot;
+                    *(uint4 *)(rc + ((size_t)(0 * 256 + ar) * 256 + br) * 4) = make_uint4(n4c[0], n4c[1], n4c[2], n4c[3]);
+                    *(uint4 *)(rc + ((size_t)(1 * 256 + ar) * 256 + br) * 4) = make_uint4(n4t[0], n4t[1], n4t[2], n4t[3]);
                 } else {
                     const Singles sa = ld_global(p.singles, A), sb = ld_global(p.singles, B);
                     uint32_t cas[9], ctl[9];

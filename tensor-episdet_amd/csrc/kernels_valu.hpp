@@ -303,6 +303,13 @@ struct BatchPair {
     int njobs;
     unsigned long long total;
     unsigned long long *queue;
+    // MODE_COUNTS (k_pairs_mfma, the plain path on the matrix cores):
+    // `rounds` rounds (A block ab, B blocks bb0 .. bb0 + rounds - 1 of 256)
+    // per launch, 8 workgroups each; round r writes its AND-counts to
+    // counts + r * slot as [class][256 A][256 B][4] (k_pairs_counts' layout)
+    int ab = 0, bb0 = 0, rounds = 0;
+    uint32_t *counts = nullptr;
+    size_t slot = 0;
 };
 
 template <int R, int MODE>

@@ -35,7 +35,8 @@ using namespace episdet;
 namespace {
 
 constexpr int BLOCK = 256;            // the reference's BLOCK_SIZE (Makefile:2)
-constexpr int PLAIN_GROUP = 16;       // rounds per FP4 count launch of the plain path
+constexpr int PLAIN_GROUP = 16;       // 3-way rounds per FP4 count launch of the plain path
+constexpr int PLAIN_GROUP2 = 32;      // 2-way rounds per FP4 count launch of the plain path
 // Words per plane row on the device: the samples rounded up to 1024 (the
 // reference's own padding), a whole chunk of every kernel that stages
 // through LDS DMA (16 or 32 words) and half a chunk of MFMA version 1, which
@@ -1524,8 +1525,8 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     // (16 workgroups per round); tile 0: the VALU count kernel, one round
     // per launch, as the reference's loop
     const bool mfma = o->tile == 1;
-    if (o->tile != 0 && !(mfma && order == 3 && mfma_ok(ds))) return EPISDET_ERR_INVALID;
-    const int group = mfma ? PLAIN_GROUP : 1;
+    if (o->tile != 0 && !(mfma && mfma_ok(ds))) return EPISDET_ERR_INVALID;
+    const int group = mfma ? (order == 3 ? PLAIN_GROUP : PLAIN_GROUP2) : 1;
     const size_t slot = (size_t)2 * 256 * 256 * (order == 3 ? 8 : 4);
     DevList dl;
     uint32_t *counts = nullptr;
@@ -1549,7 +1550,12 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     const int plain_tile = 1;   // 32 x 32 (Y, Z) per workgroup: 64 workgroups per round
     TriArgs ta;
     if (order == 3) fill_tri_common(ds, ta);
-    if (mfma) CHK(upload_single(ds, ta, 0));    // the planes and sizes the count kernel reads
+    PairArgs pa{};
+    if (mfma && order == 3) CHK(upload_single(ds, ta, 0));    // the planes and sizes the count kernel reads
+    if (mfma && order == 2) {
+        fill_pairs(ds, pa, true, 0);
+        CHK(upload_single(ds, pa, 0));
+    }
     PlainObjArgs po;
     std::memset(&po, 0, sizeof po);
     po.singles = ds->singles;
@@ -1620,8 +1626,40 @@ int run_plain(episdet_dataset *ds, const episdet_options *o, episdet_result *res
         for (int r = 0; r < cnt; r++) CHK(objective(X0 + r, yb, zb, c + slot * r, s));
         return EPISDET_OK;
     };
+    // cnt 2-way rounds (A block ab, B blocks bb0 .. bb0 + cnt - 1): one FP4
+    // count launch of the 128 x 64 pair kernel, then their objectives
+    auto mfma_pair_rounds = [&](int ab, int bb0, int cnt) -> int {
+        const int s = (int)(launches++ % NUM_STREAMS);
+        uint32_t *c = counts + slot * group * s;
+        BatchPair bp;
+        bp.jobs = (const PairArgs *)ds->jobargs;
+        bp.job_start = ds->jobstart;
+        bp.njobs = 1;
+        bp.total = 0;
+        bp.queue = nullptr;
+        bp.ab = ab;
+        bp.bb0 = bb0;
+        bp.rounds = cnt;
+        bp.counts = c;
+        bp.slot = slot;
+        hipLaunchKernelGGL((k_pairs_mfma<MODE_COUNTS>), dim3(8 * cnt), dim3(512), 0, ss.st[s], bp);
+        HIPCHK(hipGetLastError());
+        for (int r = 0; r < cnt; r++) CHK(objective(0, ab, bb0 + r, c + slot * r, s));
+        return EPISDET_OK;
+    };
     bool stop = false;
-    if (mfma) {
+    if (mfma && order == 2) {
+        for (int ab = 0; ab < nb && !stop; ab++)
+            for (int bb0 = ab; bb0 < nb; bb0 += group) {
+                long long cnt = std::min(group, nb - bb0);
+                if (limit >= 0) cnt = std::min(cnt, limit - rounds);
+                if (cnt <= 0) {
+                    stop = true;
+                    break;
+                }
+                CHK(mfma_pair_rounds(ab, bb0, (int)cnt));
+            }
+    } else if (mfma) {
         for (int yb = 0; yb < nb && !stop; yb++)
             for (int zb = yb; zb < nb && !stop; zb++) {
                 const int xend = std::min((yb + 1) * BLOCK, n);

@@ -224,9 +224,10 @@ def test_mfma_pair_versions_agree(tile):
 @pytest.mark.parametrize("tile", [0, 1])
 def test_plain_path_matches_oracle(tile):
     """The reference-shaped round loop, its rounds counted on the VALU (tile
-    0) or by the FP4 MFMA count kernel (tile 1), tables through HBM, FP64
-    objective: the oracle's winner and score, every round run; ragged
-    classes, a round count that is not a multiple of the MFMA launch's 16."""
+    0) or by the FP4 MFMA count kernels (tile 1), tables through HBM, FP64
+    objective: the oracle's winner and score, every round run, 3-way and
+    2-way; ragged classes, a round count that is not a multiple of the MFMA
+    launch's 16."""
     for n, Nc, Nt, seed in ((300, 1024, 1024, 12), (281, 1500, 1733, 14)):
         syn = episdet.generate(n, Nc, Nt, seed=seed, planted_sets=1)
         with episdet.Dataset.from_synthetic(syn) as ds:
@@ -234,9 +235,10 @@ def test_plain_path_matches_oracle(tile):
                 r = check_best(ds, syn.cases, syn.controls, Nc, Nt, 3, score, path=episdet.PATH_PLAIN, tile=tile)
                 assert r["rounds_run"] == episdet.reference_rounds(3, n) and r["tile"] == tile
             assert tuple(r["idx"]) == syn.planted[0]
+            for score in (episdet.K2, episdet.MI):
+                r = check_best(ds, syn.cases, syn.controls, Nc, Nt, 2, score, path=episdet.PATH_PLAIN, tile=tile)
+                assert r["rounds_run"] == episdet.reference_rounds(2, n) and r["tile"] == tile
             if tile == 1:
-                with pytest.raises(episdet.EpisdetError):       # the FP4 count kernel is 3-way only
-                    ds.run(2, episdet.K2, path=episdet.PATH_PLAIN, tile=1)
                 say("plain MFMA, first 37 rounds only")
                 r = ds.run(3, episdet.K2, path=episdet.PATH_PLAIN, tile=1, plain_rounds=37)
                 assert r["rounds_run"] == 37
