@@ -109,7 +109,7 @@ namespace {
 constexpr unsigned SCRATCH_CAND = 1u << 18;     // candidate slots of a no-allocation search
 
 // The regions of `scratch` one no-allocation call uses (device addresses),
-// and its two timing events.  Set for the duration of the call on its host
+// and the two timing events it uses (the dataset's).  Set for the duration of the call on its host
 // thread (t_scr); every device allocation site of the search checks it.
 struct ScratchPlan {
     Ctl *ctl = nullptr;
@@ -140,8 +140,8 @@ hipError_t dev_malloc(void **p, size_t bytes)
     return hipMalloc(p, bytes);
 }
 
-// Two timing events: the no-allocation call's own (made once per device at
-// its set-up), or two new ones destroyed with the Timer.
+// Two timing events: inside a no-allocation call the dataset's own (made
+// with the dataset), else two new ones destroyed with the Timer.
 struct Timer {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     bool own = false;
@@ -1935,12 +1935,11 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
 namespace {
 
 // Per-device set-up of the no-allocation search: the device address of
-// `scratch` (each device holds its own copy at its own address) and two
-// timing events, made once with that device current.
+// `scratch` (each device holds its own copy at its own address), taken once
+// with that device current.
 constexpr int SCRATCH_MAX_DEVICES = 64;
 struct ScratchDevice {
     unsigned char *base = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
 };
 std::mutex g_scratch_mu;      // no-allocation calls of this process run one at a time
 ScratchDevice g_scratch_dev[SCRATCH_MAX_DEVICES];
@@ -1954,8 +1953,6 @@ int scratch_device(int dev, ScratchDevice **out)
         void *ptr = nullptr;
         HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
         if (!ptr) return EPISDET_ERR_INTERNAL;
-        if (!sd.ev[0]) HIPCHK(hipEventCreate(&sd.ev[0]));
-        if (!sd.ev[1]) HIPCHK(hipEventCreate(&sd.ev[1]));
         sd.base = (unsigned char *)ptr;
     }
     *out = &sd;
@@ -2014,8 +2011,9 @@ int episdet_run_noalloc(episdet_dataset *ds, const episdet_options *opt, episdet
                      opt->order, ds->n, ds->N[0] + ds->N[1], SCRATCH_BYTES >> 20);
         return EPISDET_ERR_SCRATCH;
     }
-    sp.ev[0] = sd->ev[0];
-    sp.ev[1] = sd->ev[1];
+    // timing events: the dataset's own, made when it was created
+    sp.ev[0] = ds->ev[0];
+    sp.ev[1] = ds->ev[1];
     // Point the dataset's per-search buffers at `scratch` for this call and
     // restore them afterwards; the FP64 table and the pair table built in
     // `scratch` are forgotten, since the next call overwrites them.
