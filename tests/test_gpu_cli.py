@@ -58,7 +58,8 @@ def test_cli_errors(tmp_path):
 SHIM_BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "shim", "bin")
 
 
-@pytest.mark.parametrize("variant,order,score", [("triplets.k2", 3, O.K2), ("pairs.mi", 2, O.MI)])
+@pytest.mark.parametrize("variant,order,score", [("triplets.k2", 3, O.K2), ("triplets.mi", 3, O.MI),
+                                                  ("pairs.k2", 2, O.K2), ("pairs.mi", 2, O.MI)])
 def test_reference_entry_point_through_shim(variant, order, score):
     """A C program written against EpistasisDetectionSearch
     (search-triplets.hpp:4), linked with tests/shim/episdet_shim.cpp (the
