@@ -9,7 +9,9 @@ resident in HBM: pair-table build, the fused persistent kernel, FP64
 re-scoring of candidates and the final reduction.
 
 value      = C(n,3) * (cases + controls) / seconds per step  (the reference's
-             metric, tensor-episdet.cu:198, unpadded samples)
+             metric, tensor-episdet.cu:198, unpadded samples); seconds per
+             step = the K timed steps, bracketed by device synchronisations,
+             over K
 roofline   = the fused kernel alone, from HIP events on its stream:
              FP4-MFMA path (the default below 2^24 samples per class):
              achieved = 16 flop per 3-way set-sample (8 multiply-adds, one
@@ -141,14 +143,17 @@ def bench_batch(args, episdet):
         res, ms = episdet.run_batch(dss, orders, scores, rebuild=True)
         log(f"warmup {i}: graph {ms:.1f} ms")
     times, gms = [], []
+    episdet.synchronize()
+    t_all = time.perf_counter()
     for i in range(args.steps):
         t0 = time.perf_counter()
         res, ms = episdet.run_batch(dss, orders, scores, rebuild=True)
-        dt = time.perf_counter() - t0
-        times.append(dt)
+        times.append(time.perf_counter() - t0)
         gms.append(ms)
+    episdet.synchronize()
+    sec = (time.perf_counter() - t_all) / args.steps
+    for i, (dt, ms) in enumerate(zip(times, gms)):
         log(f"step {i}: {dt:.3f} s (graph {ms:.1f} ms)")
-    sec = max(times)
     g = sum(gms) / len(gms) / 1e3
     outside = sum(1 for r in res if r["kernel_ms"] != ms)    # jobs that ran after the graph
     if outside:
@@ -254,16 +259,19 @@ def main():
         r = step()
         log(f"warmup {i}: {r['wall_s']:.3f} s, kernel {r['kernel_ms']:.1f} ms")
     times, kms, results = [], [], []
+    episdet.synchronize()
+    t_all = time.perf_counter()
     for i in range(args.steps):
         t0 = time.perf_counter()
         r = step()
-        dt = time.perf_counter() - t0
-        times.append(dt)
+        times.append(time.perf_counter() - t0)
         kms.append(r["kernel_ms"])
         results.append(r)
+    episdet.synchronize()
+    sec = (time.perf_counter() - t_all) / args.steps
+    for i, (dt, r) in enumerate(zip(times, results)):
         log(f"step {i}: {dt:.3f} s (kernel {r['kernel_ms']:.1f} ms, pairs {r['prep_ms']:.1f} ms, rescore "
             f"{r['rescore_ms']:.2f} ms, {r['candidates']} candidates) best {r['idx'][:order]} {r['best_score']:.6f}")
-    sec = max(times)
     value = sets * N / sec
     kernel_s = sum(kms) / len(kms) / 1e3
     ran = "mfma" if results[-1]["path"] == episdet.PATH_FUSED_MFMA else "valu"

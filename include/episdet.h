@@ -278,6 +278,11 @@ const char *episdet_status_string(int status);
 /* Name, compute units and clock of device 0; EPISDET_ERR_NO_DEVICE if none. */
 int episdet_device_info(char *name, int name_len, int *compute_units, int *clock_khz);
 
+/* Waits for all work on the current device (hipDeviceSynchronize).  No
+ * counterpart in the reference; host drivers use it to bracket timed regions.
+ * Every search call above already returns only after its own work is done. */
+int episdet_device_synchronize(void);
+
 #ifdef __cplusplus
 }
 #endif

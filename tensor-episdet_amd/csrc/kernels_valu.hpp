@@ -908,11 +908,34 @@ __global__ __launch_bounds__(64) void k_rescore(RescoreArgs p)
 // read on the device (cand[0]; nothing when the overflow flag cand[1] is set
 // or the count exceeds cap: the host then searches again in windows), so it
 // follows the search on the stream with no host round trip.
-__global__ __launch_bounds__(64) void k_rescore_after(RescoreArgs p, const unsigned *cand, unsigned cap)
+// Its outcome also goes straight to pinned host memory (device-mapped), so
+// the host reads it after one stream synchronisation, with no copy: the
+// candidate count and overflow flag, and the first `slots` candidates.
+struct HostStage {
+    unsigned *ctl;                    // [0] candidate count, [1] overflow flag
+    unsigned long long *packed;       // [slots] packed indices
+    float *s32;                       // [slots] FP32 scores from the search
+    double *s64;                      // [slots] FP64 scores
+    unsigned slots;
+};
+__global__ __launch_bounds__(64) void k_rescore_after(RescoreArgs p, const unsigned *cand, unsigned cap,
+                                                      const float *cand_s32, HostStage hs)
 {
-    const unsigned count = cand[0];
-    if (cand[1] || count > cap) return;
-    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) rescore_one(p, (int)i);
+    const unsigned count = cand[0], over = cand[1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        hs.ctl[0] = count;
+        hs.ctl[1] = over;
+    }
+    if (over || count > cap) return;
+    for (unsigned i = blockIdx.x; i < count; i += gridDim.x) {
+        rescore_one(p, (int)i);
+        // lane 0 wrote the FP64 score of set i just now
+        if (threadIdx.x == 0 && i < hs.slots) {
+            hs.packed[i] = p.cand[i];
+            hs.s32[i] = cand_s32[i];
+            hs.s64[i] = p.out_s64[i];
+        }
+    }
 }
 
 // FP64 re-scoring of the candidates of many searches; the candidate counts

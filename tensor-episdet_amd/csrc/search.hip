@@ -186,7 +186,8 @@ struct episdet_dataset {
     unsigned long long *jobstart = nullptr;   // {0, units} for a batch of one
     std::vector<unsigned char> args_sent;     // what jobargs / jobstart last received (upload_single)
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};   // search start / end, re-score end
-    unsigned char *stage = nullptr;           // pinned: control block + the first candidates (STAGE_CAND)
+    unsigned char *stage = nullptr;           // pinned, device-mapped: candidate count + the first candidates
+    unsigned char *stage_dev = nullptr;       // the device's address of `stage`
     std::vector<unsigned long long> topk_packed;   // last top-k result, best first
     std::vector<double> topk_scores;
 };
@@ -887,12 +888,12 @@ int collect(episdet_dataset *ds, int count, Collected &acc)
     return collect_host(hp.data(), h32.data(), h64.data(), count, acc);
 }
 
-// Host addresses of the parts of a dataset's pinned staging area.
-Ctl *stage_ctl(episdet_dataset *ds) { return (Ctl *)ds->stage; }
-unsigned long long *stage_packed(episdet_dataset *ds) { return (unsigned long long *)(ds->stage + 256); }
-float *stage_s32(episdet_dataset *ds) { return (float *)(ds->stage + 256 + 8 * (size_t)STAGE_CAND); }
-double *stage_s64(episdet_dataset *ds) { return (double *)(ds->stage + 256 + 12 * (size_t)STAGE_CAND); }
-static_assert(sizeof(Ctl) <= 256, "control block fits the staging header");
+// The parts of a dataset's pinned staging area (k_rescore_after's HostStage),
+// by their host (base = stage) or device (base = stage_dev) addresses.
+unsigned *stage_ctl(unsigned char *base) { return (unsigned *)base; }
+unsigned long long *stage_packed(unsigned char *base) { return (unsigned long long *)(base + 256); }
+float *stage_s32(unsigned char *base) { return (float *)(base + 256 + 8 * (size_t)STAGE_CAND); }
+double *stage_s64(unsigned char *base) { return (double *)(base + 256 + 12 * (size_t)STAGE_CAND); }
 
 // Writes the gathered winner (and, for top-k, the ranking) into res and ds.
 int finish(episdet_dataset *ds, int order, const Collected &acc, episdet_result *res)
@@ -1081,34 +1082,37 @@ int launch_fused(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned long lon
 
 // The whole search in one synchronisation: the launch, then on the same
 // stream the FP64 re-scoring of the candidates it appended (count read on the
-// device) and the copy of the control block and of the first STAGE_CAND
-// candidates to the dataset's pinned staging area.  *hc receives the control
-// block; the search's and the re-scoring's device times go to *kms, *rms.
+// device), which also writes the candidate count, the overflow flag and the
+// first STAGE_CAND candidates into the dataset's pinned staging area.
+// hc->cand_count and hc->cand_over receive the count and the flag; the
+// search's and the re-scoring's device times go to *kms, *rms.
 int launch_fused_rescored(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned cap, unsigned seed_key, float *kms,
                           float *rms, Ctl *hc)
 {
     int rc = issue_fused(ds, pl, topk, 0, pl.nunits, cap, seed_key, ds->ev[0], ds->ev[1]);
     if (rc == EPISDET_OK) {
         const RescoreArgs r = rescore_args(ds, pl.order, pl.score);
+        HostStage hs;
+        hs.ctl = stage_ctl(ds->stage_dev);
+        hs.packed = stage_packed(ds->stage_dev);
+        hs.s32 = stage_s32(ds->stage_dev);
+        hs.s64 = stage_s64(ds->stage_dev);
+        hs.slots = std::min<unsigned>(cap, STAGE_CAND);
         const unsigned grid = std::min<unsigned>(cap, 16u * (unsigned)ds->cu);
-        hipLaunchKernelGGL(k_rescore_after, dim3(grid), dim3(64), 0, ds->stream, r, &ds->ctl->cand_count, cap);
+        hipLaunchKernelGGL(k_rescore_after, dim3(grid), dim3(64), 0, ds->stream, r, &ds->ctl->cand_count, cap,
+                           (const float *)ds->cand_s32, hs);
         if (hipGetLastError() != hipSuccess) rc = EPISDET_ERR_HIP;
     }
     if (rc == EPISDET_OK && hipEventRecord(ds->ev[2], ds->stream) != hipSuccess) rc = EPISDET_ERR_HIP;
-    const size_t k = std::min<size_t>(cap, STAGE_CAND);
-    if (rc == EPISDET_OK &&
-        (hipMemcpyAsync(stage_ctl(ds), ds->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, ds->stream) != hipSuccess ||
-         hipMemcpyAsync(stage_packed(ds), ds->cand, 8 * k, hipMemcpyDeviceToHost, ds->stream) != hipSuccess ||
-         hipMemcpyAsync(stage_s32(ds), ds->cand_s32, 4 * k, hipMemcpyDeviceToHost, ds->stream) != hipSuccess ||
-         hipMemcpyAsync(stage_s64(ds), ds->cand_s64, 8 * k, hipMemcpyDeviceToHost, ds->stream) != hipSuccess))
-        rc = EPISDET_ERR_HIP;
-    // wait even after an error, so no copy is left writing the staging area
+    // wait even after an error, so no kernel is left writing the staging area
     const bool synced = hipStreamSynchronize(ds->stream) == hipSuccess;
     if (rc != EPISDET_OK) return rc;
     if (!synced) return EPISDET_ERR_HIP;
     HIPCHK(hipEventElapsedTime(kms, ds->ev[0], ds->ev[1]));
     HIPCHK(hipEventElapsedTime(rms, ds->ev[1], ds->ev[2]));
-    std::memcpy(hc, stage_ctl(ds), sizeof(Ctl));
+    std::memset(hc, 0, sizeof *hc);
+    hc->cand_count = stage_ctl(ds->stage)[0];
+    hc->cand_over = stage_ctl(ds->stage)[1];
     return EPISDET_OK;
 }
 
@@ -1489,7 +1493,8 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
         // candidates are already on the host
         rms_total = rms;
         const int count = (int)hc.cand_count;
-        if (count <= (int)STAGE_CAND) CHK(collect_host(stage_packed(ds), stage_s32(ds), stage_s64(ds), count, acc));
+        if (count <= (int)STAGE_CAND)
+            CHK(collect_host(stage_packed(ds->stage), stage_s32(ds->stage), stage_s64(ds->stage), count, acc));
         else CHK(collect(ds, count, acc));
     } else {
         // more sets within the FP32 margin than slots: search again in windows
@@ -1789,6 +1794,11 @@ int episdet_device_info(char *name, int name_len, int *compute_units, int *clock
     return EPISDET_OK;
 }
 
+int episdet_device_synchronize(void)
+{
+    return hipDeviceSynchronize() == hipSuccess ? EPISDET_OK : EPISDET_ERR_HIP;
+}
+
 // Single counts, and the check that no sample is marked in both genotype
 // planes of a SNP (such input has no genotype for that sample, and the
 // derived counts of the SNP would be inconsistent).  Never captured: it
@@ -1837,10 +1847,12 @@ int episdet_dataset_create(const unsigned *cases, const unsigned *controls, int 
         rc = EPISDET_ERR_ALLOC;
     for (int e = 0; e < 3 && rc == EPISDET_OK; e++)
         if (hipEventCreate(&ds->ev[e]) != hipSuccess) rc = EPISDET_ERR_HIP;
-    if (rc == EPISDET_OK && hipHostMalloc((void **)&ds->stage, STAGE_BYTES, hipHostMallocDefault) != hipSuccess) {
+    if (rc == EPISDET_OK && hipHostMalloc((void **)&ds->stage, STAGE_BYTES, hipHostMallocMapped) != hipSuccess) {
         ds->stage = nullptr;
         rc = EPISDET_ERR_ALLOC;
     }
+    if (rc == EPISDET_OK && hipHostGetDevicePointer((void **)&ds->stage_dev, ds->stage, 0) != hipSuccess)
+        rc = EPISDET_ERR_HIP;
     for (int c = 0; c < 2 && rc == EPISDET_OK; c++) {
         const int Wv = (ds->N[c] + 31) / 32;
         ds->W[c] = ((Wv + W_ALIGN - 1) / W_ALIGN) * W_ALIGN;

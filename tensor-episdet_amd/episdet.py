@@ -99,7 +99,7 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 SEARCH_SYMBOLS = ("episdet_search", "episdet_dataset_create", "episdet_dataset_destroy", "episdet_run",
                   "episdet_run_noalloc", "episdet_run_topk", "episdet_autotune", "episdet_run_batch", "episdet_search_batch", "episdet_tables", "episdet_dump_tables",
                   "episdet_checksum", "episdet_n_choose_k", "episdet_reference_rounds", "episdet_status_string",
-                  "episdet_device_info")
+                  "episdet_device_info", "episdet_device_synchronize")
 DATA_SYMBOLS = ("episdet_gen_defaults", "episdet_generate", "episdet_padded_words", "episdet_write_dataset",
                 "episdet_read_header", "episdet_read_planes")
 
@@ -170,6 +170,8 @@ def search_lib():
         lib.episdet_status_string.restype = C.c_char_p
         lib.episdet_device_info.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         lib.episdet_device_info.restype = C.c_int
+        lib.episdet_device_synchronize.argtypes = []
+        lib.episdet_device_synchronize.restype = C.c_int
         _search = lib
     return _search
 
@@ -260,6 +262,13 @@ def device_info():
     if rc:
         raise EpisdetError(rc, "episdet_device_info")
     return name.value.decode(), cu.value, clk.value
+
+
+def synchronize():
+    """Waits for all work on the current device."""
+    rc = search_lib().episdet_device_synchronize()
+    if rc:
+        raise EpisdetError(rc, "episdet_device_synchronize")
 
 
 def search(order, score, cases, controls, num_cases, num_controls):
