@@ -168,19 +168,37 @@ __global__ __launch_bounds__(256, 2) void k_triplets_mfma4(BatchTri bt)
                                  lds_addr(bufs + ((c + G::NBUF - 1) % G::NBUF) * G::BUF_WORDS), xs);
             if (c < nch_mine) {
                 const uint32_t *cur = bufs + (c % G::NBUF) * G::BUF_WORDS;
-#pragma unroll
-                for (int s = 0; s < KW / 8; s++) {
+                // the words of slice s + 1 are read once the class-3
+                // operands of slice s are built, so their LDS latency
+                // overlaps that class's MFMAs
+                uint4 xw, yw[4], zw[2];
+                auto ld = [&](int s) {
                     const int q = 2 * s + h;
-                    const uint4 xw = *(const uint4 *)&cur[G::at(xr, q)];
-                    uint4 yw[4], zw[2];
+                    xw = *(const uint4 *)&cur[G::at(xr, q)];
 #pragma unroll
                     for (int rb = 0; rb < 4; rb++) yw[rb] = *(const uint4 *)&cur[G::at(yr[rb], q)];
                     zw[0] = *(const uint4 *)&cur[G::at(zr0, q)];
                     zw[1] = *(const uint4 *)&cur[G::at(zr1, q)];
+                };
+                ld(0);
+#pragma unroll
+                for (int s = 0; s < KW / 8; s++) {
                     m3_step<0>(xw, yw, zw, acc, bt.sh);
                     m3_step<1>(xw, yw, zw, acc, bt.sh);
                     m3_step<2>(xw, yw, zw, acc, bt.sh);
-                    m3_step<3>(xw, yw, zw, acc, bt.sh);
+                    constexpr int s3 = mf_scale<3>();
+                    const v8i_t B0 = mf_operand<3>(zw[0]), B1 = mf_operand<3>(zw[1]);
+                    v8i_t A[4];
+#pragma unroll
+                    for (int rb = 0; rb < 4; rb++) A[rb] = m2_aop<3>(xw, yw[rb], bt.sh);
+                    if (s + 1 < KW / 8) ld(s + 1);
+#pragma unroll
+                    for (int rb = 0; rb < 4; rb++) {
+                        acc[rb][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[rb], B0, acc[rb][0], 4, 4, 0, s3,
+                                                                                     0, s3);
+                        acc[rb][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(A[rb], B1, acc[rb][1], 4, 4, 0, s3,
+                                                                                     0, s3);
+                    }
                 }
             }
         }
