@@ -149,7 +149,9 @@ typedef struct {
                                    hipEvents on the launching stream             */
     double rescore_ms;          /* device time of the FP64 re-scoring            */
     double prep_ms;             /* host time of single/pair table builds in this
-                                   call (0 when cached)                          */
+                                   call (0 when cached); the single counts are
+                                   recomputed without a wait of their own, so
+                                   their device time falls in the search's      */
     double wall_s;              /* host wall-clock of this call                  */
     double fp32_max_abs_err;    /* max |FP32 - FP64| over re-scored candidates   */
     double fp32_max_rel_err;    /* same, relative to |FP64|                      */

@@ -1803,13 +1803,19 @@ int episdet_device_synchronize(void)
 // planes of a SNP (such input has no genotype for that sample, and the
 // derived counts of the SNP would be inconsistent).  Never captured: it
 // synchronises.
-static int compute_singles(episdet_dataset *ds)
+// With wait = false (a rebuild before a fused search: the planes never
+// change after the dataset passed the check) the counts are recomputed on
+// the dataset's stream with no wait of their own; the search that follows
+// on that stream reads them.  Work on other streams (the plain path's, a
+// batch graph's) needs wait = true.
+static int compute_singles(episdet_dataset *ds, bool wait = true)
 {
     unsigned *bad = &ds->ctl->bad_snp;
-    HIPCHK(hipMemsetAsync(bad, 0xFF, sizeof(unsigned), ds->stream));
+    if (wait) HIPCHK(hipMemsetAsync(bad, 0xFF, sizeof(unsigned), ds->stream));
     hipLaunchKernelGGL(k_singles, dim3(ds->n), dim3(64), 0, ds->stream, ds->planes[0], ds->W[0], ds->planes[1],
                        ds->W[1], ds->n, ds->singles, bad);
     HIPCHK(hipGetLastError());
+    if (!wait) return EPISDET_OK;
     unsigned hb = 0;
     HIPCHK(hipMemcpyAsync(&hb, bad, sizeof hb, hipMemcpyDeviceToHost, ds->stream));
     HIPCHK(hipStreamSynchronize(ds->stream));
@@ -1918,7 +1924,7 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
     std::memset(res, 0, sizeof *res);
     const double t0 = now_s();
     if (opt->rebuild) {
-        CHK(compute_singles(ds));
+        CHK(compute_singles(ds, opt->path == EPISDET_PATH_PLAIN));
         if (ds->pairs) {
             (void)hipFree(ds->pairs);
             ds->pairs = nullptr;
