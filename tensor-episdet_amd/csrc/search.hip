@@ -451,7 +451,7 @@ double key2d_host(unsigned long long k)
     return d;
 }
 
-int ensure_pairs(episdet_dataset *ds);
+int ensure_pairs(episdet_dataset *ds, bool rebuild = false);
 
 int launch_blocks(const void *kernel, int cu)
 {
@@ -738,11 +738,16 @@ int init_ctl(episdet_dataset *ds, unsigned long long u0 = 0, unsigned seed_key =
     return EPISDET_OK;
 }
 
-int ensure_pairs(episdet_dataset *ds)
+// The 3-way pair table, built once per dataset; rebuild = true builds it
+// again in place (the planes never change, so this only makes the build
+// part of a call's cost).
+int ensure_pairs(episdet_dataset *ds, bool rebuild)
 {
-    if (ds->pairs) return EPISDET_OK;
+    if (ds->pairs && !rebuild) return EPISDET_OK;
     const size_t bytes = (size_t)ds->n * ds->n * 2 * sizeof(uint4);
-    if (t_scr) {
+    if (ds->pairs) {
+        // built before: the same buffer
+    } else if (t_scr) {
         ds->pairs = t_scr->pairs;        // restored by the no-allocation call
         if (!ds->pairs) return EPISDET_ERR_INTERNAL;
     } else if (dev_malloc((void **)&ds->pairs, bytes) != hipSuccess) {
@@ -1116,7 +1121,7 @@ int launch_fused_rescored(episdet_dataset *ds, FusedPlan &pl, int topk, unsigned
     return EPISDET_OK;
 }
 
-int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune);
+int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune, bool single = false);
 
 // Order-preserving key of a float, as f2key in the kernels.
 unsigned f2key_host(float f)
@@ -1352,7 +1357,7 @@ int tune_kernel(episdet_dataset *ds, int order, int score, std::vector<episdet_t
 // tune now when asked), else the FP4-MFMA kernels wherever they are exact
 // (fewer than 2^24 samples per class, planes below 4 GiB per class) and the
 // VALU kernels otherwise.
-int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune)
+int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *tile, bool tune, bool single)
 {
     const TuneKey key{order, score, ds->n, ds->W[0], ds->W[1]};
     {
@@ -1368,6 +1373,19 @@ int choose_kernel(episdet_dataset *ds, int order, int score, int *path, int *til
         CHK(tune_kernel(ds, order, score, nullptr, path, tile));
         std::lock_guard<std::mutex> lk(g_tune_mu);
         g_tune[key] = TuneChoice{*path, *tile};
+        return EPISDET_OK;
+    }
+    // A small 2-way search on its own (not in a batch, whose graph runs MFMA
+    // kernels): the VALU kernel of 32 x 32 units spreads it over more
+    // workgroups than the MFMA kernel's 64 x 64 units, which leave most CUs
+    // idle.  Whole searches at N = 2048 through the Python API took 0.065 /
+    // 0.067 / 0.081 ms against 0.083 / 0.085 / 0.086 ms at 256 / 512 / 1024
+    // SNPs, and 0.138 against 0.109 ms at 512 SNPs x 16384 samples, so the
+    // VALU kernel is chosen below 1e9 set-samples.
+    if (single && order == 2 &&
+        (double)episdet_n_choose_k((unsigned)ds->n, 2) * (double)(ds->N[0] + ds->N[1]) < 1e9) {
+        *path = EPISDET_PATH_FUSED_VALU;
+        *tile = 1;
         return EPISDET_OK;
     }
     *path = mfma_ok(ds) ? EPISDET_PATH_FUSED_MFMA : EPISDET_PATH_FUSED_VALU;
@@ -1458,7 +1476,7 @@ int run_fused(episdet_dataset *ds, const episdet_options *o, episdet_result *res
     if (topk > EPISDET_TOPK_MAX) return EPISDET_ERR_INVALID;
     CHK(ensure_tables(ds, score));
     if (order == 3) CHK(ensure_pairs(ds));
-    if (path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, order, score, &path, &tile, o->autotune != 0));
+    if (path == EPISDET_PATH_AUTO) CHK(choose_kernel(ds, order, score, &path, &tile, o->autotune != 0, true));
     FusedPlan pl;
     CHK(plan_fused(ds, order, score, path, tile, pl));
     // Candidate slots of this search: the requested capacity, by default 1M
@@ -1925,11 +1943,7 @@ int episdet_run(episdet_dataset *ds, const episdet_options *opt, episdet_result 
     const double t0 = now_s();
     if (opt->rebuild) {
         CHK(compute_singles(ds, opt->path == EPISDET_PATH_PLAIN));
-        if (ds->pairs) {
-            (void)hipFree(ds->pairs);
-            ds->pairs = nullptr;
-        }
-        if (opt->order == 3) CHK(ensure_pairs(ds));
+        if (opt->order == 3) CHK(ensure_pairs(ds, true));
         res->prep_ms = (now_s() - t0) * 1e3;
     }
     int rc;
@@ -2181,15 +2195,9 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             ds->n < o.order || o.topk > 1)
             return EPISDET_ERR_INVALID;
         const double tp = now_s();
-        if (o.rebuild) {
-            CHK(compute_singles(ds));
-            if (ds->pairs) {
-                (void)hipFree(ds->pairs);
-                ds->pairs = nullptr;
-            }
-        }
+        if (o.rebuild) CHK(compute_singles(ds));
         CHK(ensure_tables(ds, o.score));
-        if (o.order == 3) CHK(ensure_pairs(ds));
+        if (o.order == 3) CHK(ensure_pairs(ds, o.rebuild != 0));
         CHK(ensure_cand(ds, o.candidate_cap > 0 ? (unsigned)o.candidate_cap : DEFAULT_CAP));
         results[j].prep_ms = (now_s() - tp) * 1e3;
         kpath[j] = o.path;

@@ -32,10 +32,14 @@ def test_bench_json_line():
     assert d["n_gpus"] == 1 and d["steps"] == 3 and d["warmup"] == 1
     assert d["value"] > 0 and d["ms_per_step"] > 0 and d["higher_is_better"] is True
     assert d["unit"] == "set-samples/s" and d["data"] == "synthetic" and "workload" in d["config"]
-    assert d["dtype"] == "fp4"                    # the FP4 MFMA path runs this shape
+    # AUTO runs a 2-way search this small (2.7e8 set-samples) on the VALU
+    # kernel, whose 32 x 32 units spread it over more workgroups
+    assert d["dtype"] == "u32" and d["config"]["path"] == "fused_valu"
     r = d["roofline"]
-    assert r["bound"] in ("hbm", "mfma") and r["unit"] in ("GB/s", "TFLOP/s") and r["peak"] > 0 and r["achieved"] > 0
+    assert r["bound"] == "valu" and r["unit"] == "T lane-ops/s" and r["peak"] > 0 and r["achieved"] > 0
     assert abs(r["frac"] - r["achieved"] / r["peak"]) < 1e-9 and "traffic" in r
     c = d["cpu_baseline"]
     assert c["value"] > 0 and c["cores"] >= 1 and c["kind"] in ("port", "reference") and c["sample"]
     assert d["best"]["idx"] and d["best"]["fp32_max_rel_err"] <= 1e-5
+    # the plain round loop, counted both ways
+    assert set(d["plain_path"]) == {"mfma_count", "valu_count"}
