@@ -116,15 +116,21 @@ __global__ __launch_bounds__(256, 2) void k_pairs_mfma2(BatchPair bp)
 
     int j = 0;
     for (;;) {
-        if (tid == 0) s_u = atomicAdd(bp.queue, 1ull);
+        if (tid == 0) s_u = bp.xcd ? fetch_xcd(bp.queue, bp.total) : atomicAdd(bp.queue, 1ull);
         __syncthreads();
         const unsigned long long uu = s_u;
         if (uu >= bp.total) break;
+        if (uu < bp.job_start[j]) j = 0;                  // per-XCD queues go out of order
         while (j + 1 < bp.njobs && bp.job_start[j + 1] <= uu) j++;
         const PairArgs &p = bp.jobs[j];
         const unsigned long long u = uu - bp.job_start[j];
         int tA, tB;
-        unrank_tri(u, tA, tB);
+        if (!p.blocked) {
+            unrank_tri(u, tA, tB);
+        } else if (!blocked_unrank(u, p.ntiles, tA, tB)) {
+            __syncthreads();       // every thread has read s_u before the next fetch writes it
+            continue;
+        }
         const int a0 = tA * P3_TA, b0 = tB * P3_TB;
         const int nch0 = p.W0 / P3_KW, nch = nch0 + p.W1 / P3_KW;
         const i32x4_t rs0 = m2_rsrc(p.p0, p.bytes0), rs1 = m2_rsrc(p.p1, p.bytes1);

@@ -219,7 +219,8 @@ struct PairArgs {
     const uint4 *singles;
     ScoreCtx sc;
     int ntiles;                  // tiles along the SNP axis
-    unsigned long long nunits;   // ntiles*(ntiles+1)/2
+    unsigned long long nunits;   // ntiles*(ntiles+1)/2, or blocked_units(ntiles)
+    int blocked;                 // units in XCD blocks (blocked_unrank), else triangular
     unsigned long long *queue;
     CandOut co;
     uint4 *pairs;                // MODE_PAIRTABLE: [n][n][2] uint4 (a < b only)
@@ -284,7 +285,7 @@ __device__ __forceinline__ void pair_count_class(const uint32_t *__restrict__ ba
     }
 }
 
-__device__ __forceinline__ void unrank_tri(unsigned long long u, int &ta, int &tb)
+__host__ __device__ inline void unrank_tri(unsigned long long u, int &ta, int &tb)
 {
     // u = tb*(tb+1)/2 + ta, 0 <= ta <= tb
     double d = sqrt(8.0 * (double)u + 1.0);
@@ -293,6 +294,53 @@ __device__ __forceinline__ void unrank_tri(unsigned long long u, int &ta, int &t
     while ((unsigned long long)b * (unsigned long long)(b + 1) / 2 > u) b--;
     tb = (int)b;
     ta = (int)(u - (unsigned long long)b * (unsigned long long)(b + 1) / 2);
+}
+
+// Units of a triangle of T tiles in XCD blocks.  The triangle is cut into
+// 8 x 8 blocks of units (S = ceil(T / 8) blocks a side, NB = S (S + 1) / 2 of
+// them, triangular order); unit u = 8 v + x belongs to XCD x, and XCD x
+// walks blocks x, x + 8, x + 16, ... 64 units at a time.  The 64 units one
+// XCD runs at once (one per resident workgroup) then share 8 A tiles and 8
+// B tiles, which its L2 holds.  Units outside the triangle, or in a block
+// past NB, are empty.
+__host__ __device__ inline unsigned long long blocked_units(int T)
+{
+    const unsigned long long S = (unsigned long long)((T + 7) / 8), NB = S * (S + 1) / 2;
+    return 512ull * ((NB + 7) / 8);
+}
+
+__host__ __device__ inline bool blocked_unrank(unsigned long long u, int T, int &ta, int &tb)
+{
+    const unsigned long long v = u >> 3, B = 8 * (v >> 6) + (u & 7);
+    const unsigned long long S = (unsigned long long)((T + 7) / 8);
+    if (B >= S * (S + 1) / 2) return false;
+    int sa, sb;
+    unrank_tri(B, sa, sb);
+    const int k = (int)(v & 63);
+    ta = 8 * sa + (k >> 3);
+    tb = 8 * sb + (k & 7);
+    return ta <= tb && tb < T;
+}
+
+// XCD (accelerator complex die) of the calling wave: bits [3:0] of
+// HW_REG_XCC_ID (hardware register 20).
+__device__ __forceinline__ unsigned xcc_id()
+{
+    return (unsigned)__builtin_amdgcn_s_getreg(20 | (0 << 6) | ((4 - 1) << 11)) & 7u;
+}
+
+// The next unit from per-XCD queues: q[1 + x] counts the units u = 8 v + x of
+// XCD x.  A workgroup takes its own XCD's next unit, and once those are gone
+// the other XCDs', so every queue drains; total or more means none is left.
+__device__ __forceinline__ unsigned long long fetch_xcd(unsigned long long *q, unsigned long long total)
+{
+    const unsigned x = xcc_id();
+    for (unsigned k = 0; k < 8; k++) {
+        const unsigned y = (x + k) & 7u;
+        const unsigned long long u = 8ull * atomicAdd(q + 1 + y, 1ull) + y;
+        if (u < total) return u;
+    }
+    return total;
 }
 
 // A batch of independent 2-way searches sharing one work queue (see
@@ -310,6 +358,9 @@ struct BatchPair {
     int ab = 0, bb0 = 0, rounds = 0;
     uint32_t *counts = nullptr;
     size_t slot = 0;
+    // 1: units come from the per-XCD queues queue[1..8] (fetch_xcd), which
+    // hand them out of order
+    int xcd = 0;
 };
 
 template <int R, int MODE>
@@ -1049,12 +1100,15 @@ __global__ void k_term_error(int score, int N, float invN, float k0, float c32, 
 
 // Small fill kernels (no hipMemsetAsync, so the sequence stays capturable).
 // queue0: first work unit; best0: starting global bound (key), 0 for +inf.
+// queue[0] is the work queue, queue[1..8] the per-XCD queues (fetch_xcd):
+// all start at unit queue0.
 __global__ void k_init_search(unsigned long long *queue, unsigned *best_key, unsigned *cand_count,
                               unsigned long long *best_key64, unsigned long long *checksum,
                               unsigned long long queue0, unsigned best0)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         *queue = queue0;
+        for (unsigned x = 0; x < 8; x++) queue[1 + x] = queue0 > x ? (queue0 - x + 7) / 8 : 0ull;
         *best_key = best0 ? best0 : 0xFF800000u; // key of +inf
         cand_count[0] = 0u;
         cand_count[1] = 0u;                      // overflow flag

@@ -82,6 +82,7 @@ struct ScoreTables {
 
 struct Ctl {                  // device control block
     unsigned long long queue;
+    unsigned long long xq[8];     // per-XCD queues (fetch_xcd), right after the queue
     unsigned long long best64;
     unsigned long long checksum;
     unsigned best32;
@@ -633,6 +634,7 @@ int dispatch_pairs(episdet_dataset *ds, bool mfma, int tile, PairArgs &a, hipStr
     bp.njobs = 1;
     bp.total = a.nunits;
     bp.queue = a.queue;
+    bp.xcd = mfma && tile == 1 && MODE != MODE_PAIRTABLE;
     if (mfma) {
         int per = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pairs_mfma_kernel<MODE>(tile), pairs_mfma_threads(tile),
@@ -718,7 +720,9 @@ void fill_pair_common(episdet_dataset *ds, PairArgs &a, int T)
 void fill_pairs(episdet_dataset *ds, PairArgs &a, bool mfma, int tile)
 {
     if (mfma && tile == 1) {
-        fill_pair_common(ds, a, P3_TB);     // 64 x 64 tiles, triangular order
+        fill_pair_common(ds, a, P3_TB);     // 64 x 64 tiles, triangular order or XCD blocks
+        a.blocked = a.ntiles >= 16;
+        if (a.blocked) a.nunits = blocked_units(a.ntiles);
     } else if (mfma) {
         fill_pair_common(ds, a, P2_TB);
         a.nunits = p2_units(a.ntiles);
@@ -1262,8 +1266,16 @@ unsigned long long count_sets(const episdet_dataset *ds, const FusedPlan &pl, un
         const long long TB = p2 ? P2_TB : (pl.mfma ? P3_TB : 16 * PAIR_TILES[pl.tile]);
         for (unsigned long long u = u0; u < u1; u++) {
             long long ta, tb;
-            if (p2) host_unrank_p2(u, ta, tb);
-            else host_unrank_tri(u, ta, tb);
+            if (p2) {
+                host_unrank_p2(u, ta, tb);
+            } else if (pl.mfma && pl.pa.blocked) {
+                int a, b;
+                if (!blocked_unrank(u, pl.pa.ntiles, a, b)) continue;
+                ta = a;
+                tb = b;
+            } else {
+                host_unrank_tri(u, ta, tb);
+            }
             const long long b0 = tb * TB, bhi = std::min(b0 + TB, n);
             for (long long A = ta * TA; A < std::min(ta * TA + TA, n); A++) {
                 const long long blo = std::max(b0, A + 1);
@@ -2323,7 +2335,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
     unsigned long long **d_qp = nullptr;
     unsigned **d_bp = nullptr, **d_cp = nullptr;
     constexpr int NQ = MFMA_TILES + MFMA_PAIR_TILES;   // one work queue per batched kernel
-    const unsigned long long zeros[NQ] = {};
+    constexpr int QW = 9;                              // each: the queue, then 8 per-XCD queues
+    const unsigned long long zeros[NQ * QW] = {};
     for (int tl = 0; tl < MFMA_TILES; tl++) {
         CHK(dl.put(&d_ta[tl], ta[tl].data(), ta[tl].size()));
         CHK(dl.put(&d_tstart[tl], tstart[tl].data(), tstart[tl].size()));
@@ -2332,7 +2345,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         CHK(dl.put(&d_pa[tl], pa[tl].data(), pa[tl].size()));
         CHK(dl.put(&d_pstart[tl], pstart[tl].data(), pstart[tl].size()));
     }
-    CHK(dl.put(&d_bq, zeros, NQ));
+    CHK(dl.put(&d_bq, zeros, NQ * QW));
     CHK(dl.put(&d_rj, rj.data(), rj.size()));
     CHK(dl.put(&d_qp, qp.data(), qp.size()));
     CHK(dl.put(&d_bp, bp.data(), bp.size()));
@@ -2353,8 +2366,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
         // the others' tails leave idle, then FP64 re-scoring
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         const int nb = (int)batched.size();
-        hipLaunchKernelGGL(k_init_many, dim3((std::max(nb, NQ) + 63) / 64), dim3(64), 0, st, d_qp, d_bp, d_cp, nb,
-                           d_bq, NQ);
+        hipLaunchKernelGGL(k_init_many, dim3((std::max(nb, NQ * QW) + 63) / 64), dim3(64), 0, st, d_qp, d_bp, d_cp, nb,
+                           d_bq, NQ * QW);
         HIPCHK(hipEventRecord(fork, st));
         for (int q = 0; q < NQ; q++) HIPCHK(hipStreamWaitEvent(br[q], fork, 0));
         for (int tl = 0; tl < MFMA_TILES; tl++) {
@@ -2364,7 +2377,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             bt.job_start = d_tstart[tl];
             bt.njobs = (int)tri[tl].size();
             bt.total = tstart[tl].back();
-            bt.queue = d_bq + tl;
+            bt.queue = d_bq + QW * tl;
             launch_tri_mfma<MODE_SEARCH>(tl, occupancy_blocks(mfma_kernel<MODE_SEARCH>(tl), mfma_threads(tl), cu), bt,
                                          br[tl]);
         }
@@ -2375,7 +2388,8 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             bpr.job_start = d_pstart[tl];
             bpr.njobs = (int)pairs[tl].size();
             bpr.total = pstart[tl].back();
-            bpr.queue = d_bq + MFMA_TILES + tl;
+            bpr.queue = d_bq + QW * (MFMA_TILES + tl);
+            bpr.xcd = tl == 1;
             // a failed launch shows at the end of the capture
             (void)launch_pairs_mfma<MODE_SEARCH>(
                 tl, occupancy_blocks(pairs_mfma_kernel<MODE_SEARCH>(tl), pairs_mfma_threads(tl), cu), bpr,
