@@ -7,7 +7,10 @@
 // unit; and since its cases and controls waves run side by side, the waves
 // of the smaller class finish early when the classes differ in size.  This
 // version:
-//  * work unit: 64 A SNPs x 64 B SNPs (A tile <= B tile, triangular order);
+//  * work unit: 64 A SNPs x 64 B SNPs (A tile <= B tile), in triangular
+//    order, or from 16 tiles up in XCD blocks taken from per-XCD queues
+//    (blocked_unrank, fetch_xcd), so that the units one XCD runs at once
+//    share their rows in its L2;
 //  * 256 threads = 4 waves; wave w owns A SNPs 32(w/2)..+31 (64 rows, 2 row
 //    tiles) x B SNPs 32(w%2)..+31 (64 rows, 2 column tiles), for both
 //    classes: 2 x 2 x 2 MFMA tiles, 128 accumulators;
