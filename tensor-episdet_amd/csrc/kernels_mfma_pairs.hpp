@@ -66,6 +66,32 @@ __device__ __forceinline__ void p2_unrank(unsigned long long u, int &ta, int &tb
     }
 }
 
+// The same units in XCD blocks (PairArgs::blocked): 512-SNP super-tiles on
+// both axes (4 A tiles, 8 B tiles), S = ceil(n / 512) a side, NB =
+// S (S + 1) / 2 blocks of 32 units in triangular order.  Unit u = 8 v + x
+// belongs to XCD x, which walks blocks x, x + 8, x + 16, ... 32 units at a
+// time, one per resident workgroup of this one-per-CU kernel, so the units
+// an XCD runs at once share 4 A and 8 B tiles in its L2.  Units with no
+// A < B, past the last tile, or in a block past NB are empty.
+__host__ __device__ inline unsigned long long p2_blocked_units(int n)
+{
+    const unsigned long long S = (unsigned long long)((n + 511) / 512), NB = S * (S + 1) / 2;
+    return 256ull * ((NB + 7) / 8);
+}
+
+__host__ __device__ inline bool p2_blocked_unrank(unsigned long long u, int n, int &ta, int &tb)
+{
+    const unsigned long long v = u >> 3, B = 8 * (v >> 5) + (u & 7);
+    const unsigned long long S = (unsigned long long)((n + 511) / 512);
+    if (B >= S * (S + 1) / 2) return false;
+    int sa, sb;
+    unrank_tri(B, sa, sb);
+    const int k = (int)(v & 31);
+    ta = 4 * sa + (k >> 3);
+    tb = 8 * sb + (k & 7);
+    return ta <= tb / 2 && tb < (n + P2_TB - 1) / P2_TB;
+}
+
 // Chunk loads through a scalar buffer descriptor (see m2_row_offsets): the
 // row offsets of a unit are computed once, each chunk adds its word offset
 // as the scalar soffset.  One instruction fills sixteen 64-byte LDS rows:
@@ -142,14 +168,20 @@ __global__ __launch_bounds__(512) void k_pairs_mfma(BatchPair bp)
             a0 = bp.ab * 256 + (tl >> 2) * P2_TA;
             b0 = (bp.bb0 + r) * 256 + (tl & 3) * P2_TB;
         } else {
-            if (tid == 0) s_u = atomicAdd(bp.queue, 1ull);
+            if (tid == 0) s_u = bp.xcd ? fetch_xcd(bp.queue, bp.total) : atomicAdd(bp.queue, 1ull);
             __syncthreads();
             const unsigned long long uu = s_u;
             if (uu >= bp.total) break;
+            if (uu < bp.job_start[j]) j = 0;              // per-XCD queues go out of order
             while (j + 1 < bp.njobs && bp.job_start[j + 1] <= uu) j++;
             const unsigned long long u = uu - bp.job_start[j];
             int tA, tB;
-            p2_unrank(u, tA, tB);
+            if (!bp.jobs[j].blocked) {
+                p2_unrank(u, tA, tB);
+            } else if (!p2_blocked_unrank(u, bp.jobs[j].n, tA, tB)) {
+                __syncthreads();   // every thread has read s_u before the next fetch writes it
+                continue;
+            }
             a0 = tA * P2_TA;
             b0 = tB * P2_TB;
         }

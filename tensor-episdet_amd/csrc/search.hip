@@ -634,7 +634,7 @@ int dispatch_pairs(episdet_dataset *ds, bool mfma, int tile, PairArgs &a, hipStr
     bp.njobs = 1;
     bp.total = a.nunits;
     bp.queue = a.queue;
-    bp.xcd = mfma && tile == 1 && MODE != MODE_PAIRTABLE;
+    bp.xcd = mfma && (tile == 0 || tile == 1);
     if (mfma) {
         int per = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, pairs_mfma_kernel<MODE>(tile), pairs_mfma_threads(tile),
@@ -724,8 +724,9 @@ void fill_pairs(episdet_dataset *ds, PairArgs &a, bool mfma, int tile)
         a.blocked = a.ntiles >= 16;
         if (a.blocked) a.nunits = blocked_units(a.ntiles);
     } else if (mfma) {
-        fill_pair_common(ds, a, P2_TB);
-        a.nunits = p2_units(a.ntiles);
+        fill_pair_common(ds, a, P2_TB);     // 128 x 64 tiles, B-tile-major or XCD blocks
+        a.blocked = a.ntiles >= 16;
+        a.nunits = a.blocked ? p2_blocked_units(ds->n) : p2_units(a.ntiles);
     } else {
         fill_pair_common(ds, a, 16 * PAIR_TILES[tile]);
     }
@@ -1266,7 +1267,12 @@ unsigned long long count_sets(const episdet_dataset *ds, const FusedPlan &pl, un
         const long long TB = p2 ? P2_TB : (pl.mfma ? P3_TB : 16 * PAIR_TILES[pl.tile]);
         for (unsigned long long u = u0; u < u1; u++) {
             long long ta, tb;
-            if (p2) {
+            if (p2 && pl.pa.blocked) {
+                int a, b;
+                if (!p2_blocked_unrank(u, (int)n, a, b)) continue;
+                ta = a;
+                tb = b;
+            } else if (p2) {
                 host_unrank_p2(u, ta, tb);
             } else if (pl.mfma && pl.pa.blocked) {
                 int a, b;
@@ -2389,7 +2395,7 @@ int episdet_run_batch(episdet_dataset *const *dss, const episdet_options *opts, 
             bpr.njobs = (int)pairs[tl].size();
             bpr.total = pstart[tl].back();
             bpr.queue = d_bq + QW * (MFMA_TILES + tl);
-            bpr.xcd = tl == 1;
+            bpr.xcd = tl == 0 || tl == 1;
             // a failed launch shows at the end of the capture
             (void)launch_pairs_mfma<MODE_SEARCH>(
                 tl, occupancy_blocks(pairs_mfma_kernel<MODE_SEARCH>(tl), pairs_mfma_threads(tl), cu), bpr,
