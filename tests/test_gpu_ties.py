@@ -90,15 +90,16 @@ def test_two_way_all_sets_tied(path, tile):
                 assert r["fallback"] == 1
 
 
-def planted_with_copies(n, Nc, Nt, seed, copies):
-    """A planted triplet whose SNPs are each copied `copies` times further
-    up: every combination of copies ties with the planted set exactly, so
-    (copies + 1)^3 sets share the best score."""
-    syn = episdet.generate(n, Nc, Nt, seed=seed, planted_sets=1, planted_order=3)
+def planted_with_copies(n, Nc, Nt, seed, copies, order=3):
+    """A planted set (triplet, or pair with order=2) whose SNPs are each
+    copied `copies` times further up: every combination of copies ties with
+    the planted set exactly, so (copies + 1)^order sets share the best
+    score."""
+    syn = episdet.generate(n, Nc, Nt, seed=seed, planted_sets=1, planted_order=order)
     cases, controls = syn.cases.copy(), syn.controls.copy()
-    a, b, c = syn.planted[0]
-    free = [v for v in range(n) if v not in (a, b, c)][::-1]
-    for src in (a, b, c):
+    planted = syn.planted[0]
+    free = [v for v in range(n) if v not in planted][::-1]
+    for src in planted:
         for _ in range(copies):
             d = free.pop(0)
             cases[d], controls[d] = cases[src], controls[src]
@@ -141,6 +142,33 @@ def test_topk_windows_keep_the_ranking():
                     assert s == [float(x) for x in ref_s[:k]]
                     if k > 1:                      # k = 1 is the single-best search
                         assert r["fallback"] == 1
+
+
+@pytest.mark.parametrize("tile", [0, 1])
+def test_two_way_windows_in_xcd_blocks(tile):
+    """2-way on 1100 SNPs (18 tiles of 64, so both MFMA pair kernels take
+    their units in XCD blocks from per-XCD queues) with 101^2 = 10201 exact
+    K2 ties at the top, more than the 8192 candidate slots asked for: single
+    best (candidate_cap=64, raised to one unit's sets) and top-k
+    (candidate_cap=8192) rerun in windows of units, and return the oracle's
+    winner and ranking prefix, for K2 and for MI."""
+    Nc, Nt = 1200, 1100
+    cases, controls = planted_with_copies(1100, Nc, Nt, seed=47, copies=100, order=2)
+    with episdet.Dataset(cases, controls, Nc, Nt) as ds:
+        for score in (episdet.K2, episdet.MI):
+            ref_p, ref_s = O.ranked(cases, controls, Nc, Nt, 2, score)
+            say(f"2-way windows: tile={tile} score={score} single best")
+            r = ds.run(2, score, path=M, tile=tile, candidate_cap=64)
+            assert r["packed"] == int(ref_p[0]) and r["best_score"] == float(ref_s[0]), r
+            if score == episdet.K2:            # the 10201 ties lead the K2 ranking
+                assert r["fallback"] == 1
+            for k in (100, 1000):
+                say(f"2-way windows: tile={tile} score={score} k={k}")
+                pk, sc, rk = ds.topk(2, k, score=score, path=M, tile=tile, candidate_cap=8192)
+                assert pk == [int(x) for x in ref_p[:k]]
+                assert sc == [float(x) for x in ref_s[:k]]
+                if score == episdet.K2:
+                    assert rk["fallback"] == 1
 
 
 def test_topk_all_sets_tied():
