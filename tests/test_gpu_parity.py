@@ -309,14 +309,14 @@ def test_duplicate_snps_tie_rule():
 
 # --------------------------------------------- reference shapes (small) --
 
-def test_config1_all_pair_tables():
+def test_512_snps_2048_samples_all_pair_tables():
     """512 SNPs, 1024 cases + 1024 controls: all 130,816 pair tables
     bit-exact and the best pair identical."""
     syn = episdet.generate(512, 1024, 1024, seed=1)
     with episdet.Dataset.from_synthetic(syn) as ds:
         ref, cs = O.all_tables(syn.cases, syn.controls, 1024, 1024, 2)
         for path, tile in ((episdet.PATH_FUSED_VALU, 0), (episdet.PATH_FUSED_MFMA, 0), (episdet.PATH_FUSED_MFMA, 1)):
-            say(f"config1 dump path={path} tile={tile}")
+            say(f"512 x 2048 dump path={path} tile={tile}")
             dump = ds.dump_tables(2, path=path, tile=tile)
             assert dump.shape[0] == 130_816
             assert (dump == ref).all()
